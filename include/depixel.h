@@ -1,0 +1,224 @@
+/*
+ * depixel.h -- C ABI of the MI355X-native depixelization path.
+ *
+ * Drop-in for the per-frame call sequence of marcoc2/pixel-art-remaster-gpu
+ * (kernel.cu:286-526, `launch_kernel`), plus the stage entry points that
+ * sequence is made of, a batched path and a serving loop.  Plain pointers and
+ * sizes only; HIP streams travel as `void*` (a hipStream_t, NULL = default).
+ *
+ * Conventions
+ *   - every entry point except launch_kernel returns a dpx_status; the
+ *     message of the last failure on the calling thread is dpx_last_error();
+ *   - the caller owns every host buffer and every device buffer passed in;
+ *     the library owns the device memory behind a dpx_context / dpx_server;
+ *   - image buffers: 8-bit pixels, `channels` 3 (the reference's BGR order
+ *     unless DPX_RGB) or 4 (BGRA/RGBA, alpha extension), `stride` bytes per
+ *     row, rows bottom-up (the reference's flipped OpenCV layout,
+ *     Image.cpp:21-38) unless DPX_TOPDOWN;
+ *   - output images: RGBA8, (scale*W) x (scale*H), same row order as the input;
+ *   - graph: 1 byte per pixel, bit e = similar to neighbour e
+ *     (e -> (dx,dy): 0:(-1,+1) 1:(0,+1) 2:(+1,+1) 3:(-1,0) 4:(+1,0)
+ *      5:(-1,-1) 6:(0,-1) 7:(+1,-1); +1 = next row up);
+ *   - diagram: dpx_point[W*H*DPX_CELL_SIZE] cell slots, edge_count int32[W*H].
+ */
+#ifndef DEPIXEL_H
+#define DEPIXEL_H
+
+#include <stdbool.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Point (point.cu:2-11): 8-byte {float x, y}, layout-compatible with float2 */
+typedef struct { float x, y; } dpx_point;
+/* uchar4 of the colour VBO (kernel.cu:67-104) */
+typedef struct { uint8_t x, y, z, w; } dpx_uchar4;
+
+#define DPX_CELL_SIZE 45   /* slots per cell, CELL_SIZE kernel.cu:8 */
+
+/* layout / semantics flags */
+#define DPX_RGB      1u    /* byte order R,G,B(,A) instead of B,G,R(,A)              */
+#define DPX_TOPDOWN  2u    /* buffer row 0 is the top row (output follows the input) */
+#define DPX_ALPHA    4u    /* channels==4: alpha-0 pixels are their own colour class */
+
+typedef enum {
+    DPX_OK = 0,
+    DPX_EINVAL = -1,     /* bad argument (sizes, pointers, flags)              */
+    DPX_ENOMEM = -2,     /* device or host allocation failed                   */
+    DPX_EHIP = -3,       /* a HIP runtime call failed                          */
+    DPX_ENODEV = -4,     /* no usable GPU                                      */
+    DPX_EBUDGET = -5,    /* request larger than the server's HBM budget        */
+    DPX_ETIMEOUT = -6,   /* wait timed out (request still pending)             */
+    DPX_ECANCELLED = -7, /* request cancelled                                  */
+    DPX_ECLOSED = -8,    /* server or stream closed                            */
+    DPX_EINJECTED = -9   /* failure injected by dpx_server_inject_failure      */
+} dpx_status;
+
+typedef struct {
+    int32_t scale;       /* output pixels per input pixel, 1..32 (batched path: 1..15) */
+    int32_t subdivide;   /* corner-cutting smoothing on/off (simpleVBO.cpp:41)       */
+    uint32_t flags;      /* DPX_RGB | DPX_TOPDOWN | DPX_ALPHA                         */
+    int32_t reserved;    /* must be 0                                                 */
+} dpx_params;
+
+const char* dpx_last_error(void);
+const char* dpx_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Drop-in: replaces launch_kernel, kernel.cu:286-288 (declared for its one  */
+/* caller at simpleVBO.cpp:57-60, called from runCuda simpleVBO.cpp:151).    */
+/* Same arguments and meaning:                                               */
+/*   pos, colorPos  device buffers of W*H*45 entries (or NULL to skip the    */
+/*                  VBO packing of kernel.cu:463-475)                         */
+/*   time           unused, as in the reference                               */
+/*   img_data       host BGR8, bottom-up, stride img_widthstep                */
+/*   edge_count_h   host int[W*H], written                                    */
+/*   graph_h        host char[W*H], written with the resolved graph           */
+/* Returns a malloc'd host dpx_point[W*H*45] that the caller frees (as       */
+/* simpleVBO.cpp:437 does).  Where the reference prints and exit(-1)s on a   */
+/* CUDA error (kernel.cu:39-62), this returns NULL and sets dpx_last_error.  */
+/* ------------------------------------------------------------------------ */
+dpx_point* launch_kernel(dpx_point* pos, dpx_uchar4* colorPos, float time, char* img_data,
+                         int img_width, int img_height, int img_widthstep, int* edge_count_h,
+                         char* graph_h, bool subdivide);
+
+/* ------------------------------------------------------------------------ */
+/* Plain per-frame path: the reference's sequence one stage per kernel with  */
+/* intermediates in HBM, per-call device allocation and blocking copies,     */
+/* plus the raster that replaces glDrawArrays (simpleVBO.cpp:236-285).       */
+/* Any of out_rgba / graph_out / edge_count_out / diagram_out may be NULL.   */
+/* ------------------------------------------------------------------------ */
+int dpx_frame(const uint8_t* img, int width, int height, int stride, int channels,
+              const dpx_params* params, uint8_t* out_rgba, int out_stride, uint8_t* graph_out,
+              int32_t* edge_count_out, dpx_point* diagram_out);
+
+/* ------------------------------------------------------------------------ */
+/* Stage entry points on caller-owned device buffers, in kernel.cu's order.  */
+/* All enqueue on `stream` and return without synchronising.                */
+/* ------------------------------------------------------------------------ */
+/* graph_Kernel kernel.cu:140-159 */
+int dpx_stage_similarity_graph(const uint8_t* d_img, int width, int height, int stride,
+                               int channels, uint32_t flags, uint8_t* d_graph, void* stream);
+/* trivial_cross_Kernel kernel.cu:162-177 (in place) */
+int dpx_stage_trivial_cross(uint8_t* d_graph, int width, int height, void* stream);
+/* snapshot kernel.cu:415 + ambiguous_cross_Kernel kernel.cu:180-189 (in place;
+ * d_graph_aux receives the snapshot) */
+int dpx_stage_heuristic_cross(uint8_t* d_graph, uint8_t* d_graph_aux, int width, int height,
+                              void* stream);
+/* cells_Kernel kernel.cu:192-213 */
+int dpx_stage_cells(const uint8_t* d_graph, int width, int height, dpx_point* d_diagram,
+                    int32_t* d_edge_count, void* stream);
+/* snapshots kernel.cu:434-437 + subdivision_Kernel kernel.cu:216-261 */
+int dpx_stage_subdivide(const uint8_t* d_img, int width, int height, int stride, int channels,
+                        uint32_t flags, const uint8_t* d_graph, dpx_point* d_diagram,
+                        dpx_point* d_diagram_aux, int32_t* d_edge_count,
+                        int32_t* d_edge_count_aux, void* stream);
+/* triangulate_Kernel kernel.cu:264-282 */
+int dpx_stage_triangulate(dpx_point* d_diagram, const int32_t* d_edge_count, int width,
+                          int height, void* stream);
+/* color_kernel kernel.cu:67-104 + position_kernel kernel.cu:109-137 */
+int dpx_stage_export_vbo(const uint8_t* d_img, int width, int height, int stride, int channels,
+                         uint32_t flags, const dpx_point* d_diagram,
+                         const int32_t* d_edge_count, dpx_point* d_pos, dpx_uchar4* d_color,
+                         void* stream);
+/* glDrawArrays(GL_TRIANGLES) simpleVBO.cpp:281 with glScalef(scale) (258) */
+int dpx_stage_rasterize(const uint8_t* d_img, int width, int height, int stride, int channels,
+                        uint32_t flags, const dpx_point* d_diagram,
+                        const int32_t* d_edge_count, int scale, uint8_t* d_out,
+                        int out_stride, void* stream);
+/* components of the similarity graph (the reference's border walk,
+ * cc_functions.cu:348-503, is off its live path): labels[k] = smallest node
+ * index of k's component */
+int dpx_stage_components(const uint8_t* d_graph, int width, int height, int32_t* d_labels,
+                         void* stream);
+
+/* ------------------------------------------------------------------------ */
+/* Batched path: many frames / sprites of uneven sizes through one fused     */
+/* kernel chain.  Inputs and outputs live in caller-owned device arenas.     */
+/* No counterpart in the reference (it renders one frame per call).          */
+/* ------------------------------------------------------------------------ */
+typedef struct {
+    uint64_t in_offset;   /* byte offset of the frame in d_in                  */
+    uint64_t out_offset;  /* byte offset of the RGBA output in d_out           */
+    int32_t width, height, stride, channels;
+    int32_t scale;        /* 1..15                                             */
+    int32_t subdivide;
+    uint32_t flags;
+    int32_t out_stride;   /* bytes per output row (>= 4*scale*width)           */
+} dpx_item;
+
+typedef struct dpx_context dpx_context;
+
+int dpx_context_create(int device, dpx_context** out);
+void dpx_context_destroy(dpx_context* ctx);
+/* Enqueue the batch on `stream`; returns once enqueued.  The context's
+ * scratch (graph buffers, descriptors) is reused across calls: calls on one
+ * context must be serialised by the caller (same stream or explicit sync). */
+int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_out, const dpx_item* items,
+                  int n_items, void* stream);
+/* Optional resolved graphs of the last batch (device pointer + per-item
+ * offsets), for parity checks. */
+int dpx_batch_graph(dpx_context* ctx, const uint8_t** d_graph, const uint64_t** offsets_host);
+
+/* Per-kernel timing of the batched path: HIP events recorded on the batch's
+ * stream around each of its kernels.  Off by default; reading waits for the
+ * recorded batches, returns the accumulated times and resets them. */
+typedef struct {
+    double graph_ms, heuristic_ms, raster_ms;  /* summed over the batches        */
+    uint64_t batches;                            /* batches recorded since reset   */
+} dpx_batch_times;
+int dpx_batch_timing(dpx_context* ctx, int enable);
+int dpx_batch_timing_read(dpx_context* ctx, dpx_batch_times* out);
+
+/* ------------------------------------------------------------------------ */
+/* Serving: continuous batching of frames from many streams under an HBM     */
+/* budget.  submit is thread-safe; results land in caller host buffers.      */
+/* ------------------------------------------------------------------------ */
+typedef struct dpx_server dpx_server;
+
+typedef struct {
+    uint64_t submitted, completed, failed, rejected, cancelled;
+    uint64_t batches;
+    double frames_per_s;     /* completed frames / wall time since create  */
+    double p50_ms, p99_ms;   /* submit -> completion latency               */
+    double mean_batch;       /* frames per device batch                    */
+    uint64_t hbm_in_use, hbm_peak, hbm_budget;
+} dpx_server_stats;
+
+int dpx_server_create(int device, uint64_t hbm_budget_bytes, int max_batch, dpx_server** out);
+void dpx_server_destroy(dpx_server* srv);
+int dpx_server_open_stream(dpx_server* srv, int width, int height, int channels,
+                           const dpx_params* params, int* stream_id);
+int dpx_server_close_stream(dpx_server* srv, int stream_id);
+/* img: host frame (stride bytes/row); out: host RGBA buffer kept alive until
+ * the ticket completes. */
+int dpx_server_submit(dpx_server* srv, int stream_id, const uint8_t* img, int stride,
+                      uint8_t* out, int out_stride, uint64_t* ticket);
+int dpx_server_wait(dpx_server* srv, uint64_t ticket, int timeout_ms);
+int dpx_server_cancel(dpx_server* srv, uint64_t ticket);
+int dpx_server_stats_get(dpx_server* srv, dpx_server_stats* out);
+/* test hook: make the k-th submitted request (0-based) fail on the device side */
+int dpx_server_inject_failure(dpx_server* srv, uint64_t request_index);
+
+/* ------------------------------------------------------------------------ */
+/* Synthetic inputs (deterministic, splitmix64): the workloads the path is   */
+/* measured on.  Host-only helpers.                                          */
+/*   kind 0: 256x224-style frame: 16-colour palette, 40 rectangles, 20 1-px  */
+/*           staircases, 4 checkerboard-dither regions, 30 isolated pixels   */
+/*   kind 1: sprite with alpha: 4-12 colours, dark 1-px outline, ~30%        */
+/*           transparent background, interior dither (channels must be 4)    */
+/*   kind 2: console frame: 16-64 colours, rectangles + 1-px random walks    */
+/*   kind 4: large frame: 256 colours, 25% dither, diagonal lattice          */
+/* ------------------------------------------------------------------------ */
+int dpx_synth_frame(int kind, uint64_t seed, int width, int height, int channels, uint8_t* out,
+                    int stride);
+/* sprite size of sprite `index` of seed `seed` (sizes of the sprite mix) */
+int dpx_synth_sprite_size(uint64_t seed, int index, int* width, int* height);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DEPIXEL_H */

@@ -1,0 +1,33 @@
+// common.hpp -- status codes, error reporting and launch helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdint.h>
+
+#include "../../include/depixel.h"
+
+namespace dpx {
+
+// Records a message as the calling thread's last error and returns `code`.
+int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+// Returns DPX_ENODEV when the process sees no GPU.
+int require_device();
+
+}  // namespace dpx
+
+#define DPX_HIP(call)                                                                                   \
+    do {                                                                                                \
+        hipError_t dpx_e_ = (call);                                                                     \
+        if (dpx_e_ != hipSuccess)                                                                       \
+            return ::dpx::fail(dpx_e_ == hipErrorOutOfMemory ? DPX_ENOMEM : DPX_EHIP, "%s: %s (%s:%d)", \
+                               #call, hipGetErrorString(dpx_e_), __FILE__, __LINE__);                   \
+    } while (0)
+
+#define DPX_CHECK(cond, ...)                          \
+    do {                                              \
+        if (!(cond)) return ::dpx::fail(__VA_ARGS__); \
+    } while (0)

@@ -1,0 +1,669 @@
+// fused.hip -- the batched path: many frames or sprites of uneven size
+// through three fused kernels.
+//
+//   kf_graph   LDS tile 64x16 (+2 halo): YUV once per pixel, 8 similarity bits,
+//              trivial crossing removal -> snapshot graph (1 B/px)
+//   kf_heur    heuristic crossing resolution on the snapshot -> final graph
+//   kf_raster  tile of 32x8 cells: cell keys (and, with subdivision, hulls,
+//              smoothed polygons and their coverage) in LDS, then every output
+//              row segment of a cell resolves its S pixels against the 3x3
+//              candidate cells in draw order and streams RGBA8 to HBM.
+//
+// Coverage of an unsmoothed cell depends only on its 12-bit key and the
+// scale, so it is precomputed once per scale into a table of sample-row
+// bitmasks (kt_mask): bit b of row r of key k says whether sample
+// (kmin+b, kmin+r) of the cell's local sample grid is covered, kmin =
+// -(S/2)-1, MW = 2S+2 rows and columns (local [-0.5, 1.5] of the cell).
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+#include <map>
+#include <mutex>
+#include <vector>
+
+#include "common.hpp"
+#include "geom.hpp"
+#include "graph.hpp"
+#include "pixel.hpp"
+
+namespace dpx {
+
+constexpr int kMaxScale = 15;
+constexpr int kKeys = 4096;
+constexpr int kHullBytes = 32;  // n, x[8], y[8] (quarter units), padding
+
+__host__ __device__ constexpr int mask_kmin(int S) { return -(S / 2) - 1; }
+__host__ __device__ constexpr int mask_w(int S) { return 2 * S + 2; }
+
+struct DevItem {
+    uint64_t in_off, out_off, g_off;
+    int32_t W, H, stride, ch;
+    uint32_t flags;
+    int32_t out_stride, scale, subdivide;
+    int32_t aligned16;  // output rows 16-byte aligned
+    int32_t pad[3];
+};
+
+// ---------------------------------------------------------------------------
+// tables
+// ---------------------------------------------------------------------------
+__global__ void kt_hull(int8_t* __restrict__ table)
+{
+    int key = blockIdx.x * blockDim.x + threadIdx.x;
+    if (key >= kKeys) return;
+    int hx[2 * kMaxPts], hy[2 * kMaxPts];
+    int n = cell_hull((uint32_t)key, hx, hy);
+    int8_t* e = table + (size_t)key * kHullBytes;
+    e[0] = (int8_t)n;
+    for (int t = 0; t < 8; t++) {
+        e[1 + t] = (int8_t)(t < n ? hx[t] : 0);
+        e[9 + t] = (int8_t)(t < n ? hy[t] : 0);
+    }
+}
+
+// Coverage bits of one sample row of a polygon given in 1/64 units and
+// already ear-clipped (tri, nt).  Units of the test: 1/(128 S) of a cell.
+__device__ inline uint32_t coverage_row(const int* x, const int* y, const uint8_t* tri, int nt, int S, int r)
+{
+    const int kmin = mask_kmin(S), mw = mask_w(S);
+    int sy = (2 * (kmin + r) + 1) * 64;
+    uint32_t bits = 0;
+    for (int t = 0; t < nt; t++) {
+        int ax = x[tri[3 * t]] * 2 * S, ay = y[tri[3 * t]] * 2 * S;
+        int bx = x[tri[3 * t + 1]] * 2 * S, by = y[tri[3 * t + 1]] * 2 * S;
+        int cx = x[tri[3 * t + 2]] * 2 * S, cy = y[tri[3 * t + 2]] * 2 * S;
+        int ymin = min(ay, min(by, cy)), ymax = max(ay, max(by, cy));
+        if (sy < ymin || sy > ymax) continue;
+        for (int b = 0; b < mw; b++) {
+            int sx = (2 * (kmin + b) + 1) * 64;
+            if (tri_covers(ax, ay, bx, by, cx, cy, sx, sy)) bits |= 1u << b;
+        }
+    }
+    return bits;
+}
+
+__global__ void kt_mask(const int8_t* __restrict__ hull_table, int S, uint32_t* __restrict__ mask)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    const int mw = mask_w(S);
+    if (gid >= kKeys * mw) return;
+    int key = gid / mw, r = gid % mw;
+    const int8_t* e = hull_table + (size_t)key * kHullBytes;
+    int n = e[0];
+    int x[kMaxPoly], y[kMaxPoly];
+    for (int t = 0; t < n; t++) { x[t] = 16 * e[1 + t]; y[t] = 16 * e[9 + t]; }
+    uint8_t tri[3 * kMaxPoly];
+    int ok;
+    int nt = ear_clip(x, y, n, tri, &ok);
+    mask[gid] = coverage_row(x, y, tri, nt, S, r);
+}
+
+// ---------------------------------------------------------------------------
+// work distribution: block b -> (item, tile) through a prefix of tile counts
+// ---------------------------------------------------------------------------
+__device__ inline int find_item(const uint32_t* __restrict__ prefix, int n, uint32_t b)
+{
+    int lo = 0, hi = n;  // prefix[lo] <= b < prefix[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (prefix[mid] <= b) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// ---------------------------------------------------------------------------
+// kf_graph: similarity graph + trivial crossing removal
+// ---------------------------------------------------------------------------
+constexpr int GTX = 64, GTY = 16;
+constexpr int GLX = GTX + 4, GLY = GTY + 4;  // YUV tile, halo 2
+constexpr int GBX = GTX + 2, GBY = GTY + 2;  // bit tile, halo 1
+
+__global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix,
+                                                int n_items, const uint8_t* __restrict__ in, uint8_t* __restrict__ gaux)
+{
+    __shared__ uint32_t s_yuv[GLX * GLY];
+    __shared__ uint8_t s_cls[GLX * GLY];  // 0 opaque, 1 transparent, 2 outside the frame
+    __shared__ uint8_t s_bits[GBX * GBY];
+    const int t = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const int ii = find_item(prefix, n_items, b);
+    const DevItem it = items[ii];
+    const int tiles_x = (it.W + GTX - 1) / GTX;
+    const int tile = (int)(b - prefix[ii]);
+    const int x0 = (tile % tiles_x) * GTX, y0 = (tile / tiles_x) * GTY;
+    ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags };
+
+    for (int k = t; k < GLX * GLY; k += 256) {
+        int x = x0 - 2 + k % GLX, y = y0 - 2 + k / GLX;
+        if (x >= 0 && y >= 0 && x < it.W && y < it.H) {
+            Yuv v = yuv_of(load_px(im, x, y), it.flags, it.ch);
+            s_yuv[k] = v.w;
+            s_cls[k] = (uint8_t)v.transparent;
+        } else {
+            s_cls[k] = 2;
+        }
+    }
+    __syncthreads();
+    for (int k = t; k < GBX * GBY; k += 256) {
+        int lx = k % GBX + 1, ly = k / GBX + 1;
+        int c = ly * GLX + lx;
+        uint32_t bits = 0;
+        if (s_cls[c] != 2) {
+            Yuv a{ s_yuv[c], s_cls[c] };
+#pragma unroll
+            for (int e = 0; e < 8; e++) {
+                int q = c + edy(e) * GLX + edx(e);
+                if (s_cls[q] == 2) continue;
+                Yuv bb{ s_yuv[q], s_cls[q] };
+                if (similar(a, bb)) bits |= 1u << e;
+            }
+        }
+        s_bits[k] = (uint8_t)bits;
+    }
+    __syncthreads();
+    uint8_t* g = gaux + it.g_off;
+    for (int k = t; k < GTX * GTY; k += 256) {
+        int lx = k % GTX, ly = k / GTX;
+        int x = x0 + lx, y = y0 + ly;
+        if (x >= it.W || y >= it.H) continue;
+        int c = (ly + 1) * GBX + lx + 1;
+        uint32_t own = s_bits[c], nb[8];
+#pragma unroll
+        for (int e = 0; e < 8; e++) nb[e] = s_bits[c + edy(e) * GBX + edx(e)];
+        g[(size_t)y * it.W + x] = (uint8_t)(own & ~trivial_clear_mask(own, nb));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kf_heur: heuristic crossing resolution on the snapshot
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void kf_heur(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix,
+                                               int n_items, const uint8_t* __restrict__ gaux, uint8_t* __restrict__ gfin)
+{
+    __shared__ uint8_t s_g[GBX * GBY];
+    const int t = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const int ii = find_item(prefix, n_items, b);
+    const DevItem it = items[ii];
+    const int tiles_x = (it.W + GTX - 1) / GTX;
+    const int tile = (int)(b - prefix[ii]);
+    const int x0 = (tile % tiles_x) * GTX, y0 = (tile / tiles_x) * GTY;
+    const uint8_t* aux = gaux + it.g_off;
+    for (int k = t; k < GBX * GBY; k += 256) {
+        int x = x0 - 1 + k % GBX, y = y0 - 1 + k / GBX;
+        s_g[k] = (x >= 0 && y >= 0 && x < it.W && y < it.H) ? aux[(size_t)y * it.W + x] : 0;
+    }
+    __syncthreads();
+    uint8_t* g = gfin + it.g_off;
+    for (int k = t; k < GTX * GTY; k += 256) {
+        int lx = k % GTX, ly = k / GTX;
+        int x = x0 + lx, y = y0 + ly;
+        if (x >= it.W || y >= it.H) continue;
+        int c = (ly + 1) * GBX + lx + 1;
+        uint32_t own = s_g[c];
+        uint32_t out = own;
+        if (own & 0xA5u) {  // any diagonal bit (0, 2, 5, 7)
+            uint32_t nb[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) nb[e] = s_g[c + edy(e) * GBX + edx(e)];
+            out = own & ~heuristic_clear_mask(aux, it.W, x, y, own, nb);
+        }
+        g[(size_t)y * it.W + x] = (uint8_t)out;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kf_raster: cells (+ subdivision) + gather raster
+// ---------------------------------------------------------------------------
+constexpr int RTX = 32, RTY = 8;
+constexpr int RGX = RTX + 6, RGY = RTY + 6;  // graph bytes, halo 3
+constexpr int RHX = RTX + 4, RHY = RTY + 4;  // hulls, halo 2 (subdivision)
+constexpr int RCX = RTX + 2, RCY = RTY + 2;  // candidate cells, halo 1
+constexpr uint16_t kNoCell = 0xFFFF;
+
+template <int S, bool SUB>
+__global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const int32_t* __restrict__ group,
+                                                 const uint32_t* __restrict__ prefix, int n_group,
+                                                 const uint8_t* __restrict__ in, const uint8_t* __restrict__ gfin,
+                                                 const int8_t* __restrict__ hull_table,
+                                                 const uint32_t* __restrict__ mask_table, uint8_t* __restrict__ out)
+{
+    constexpr int MW = mask_w(S), KMIN = mask_kmin(S);
+    __shared__ uint8_t s_g[RGX * RGY];
+    __shared__ uint16_t s_key[RCX * RCY];
+    __shared__ uint32_t s_col[RCX * RCY];
+    __shared__ uint32_t s_mask[RCX * RCY * MW];
+    __shared__ int8_t s_hull[SUB ? RHX * RHY * 17 : 1];
+
+    const int t = threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const int gi = find_item(prefix, n_group, b);
+    const DevItem it = items[group[gi]];
+    const int tiles_x = (it.W + RTX - 1) / RTX;
+    const int tile = (int)(b - prefix[gi]);
+    const int x0 = (tile % tiles_x) * RTX, y0 = (tile / tiles_x) * RTY;
+    ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags };
+    const uint8_t* g = gfin + it.g_off;
+
+    for (int k = t; k < RGX * RGY; k += 256) {
+        int x = x0 - 3 + k % RGX, y = y0 - 3 + k / RGX;
+        s_g[k] = (x >= 0 && y >= 0 && x < it.W && y < it.H) ? g[(size_t)y * it.W + x] : 0;
+    }
+    __syncthreads();
+    // candidate cells: key, colour and the table's coverage rows
+    for (int k = t; k < RCX * RCY; k += 256) {
+        int lx = k % RCX, ly = k / RCX;
+        int x = x0 - 1 + lx, y = y0 - 1 + ly;
+        if (x >= 0 && y >= 0 && x < it.W && y < it.H) {
+            int c = (ly + 2) * RGX + lx + 2;
+            uint32_t key = cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
+            s_key[k] = (uint16_t)key;
+            s_col[k] = cell_rgba(load_px(im, x, y), it.ch);
+            const uint32_t* row = mask_table + (size_t)key * MW;
+#pragma unroll
+            for (int r = 0; r < MW; r++) s_mask[k * MW + r] = row[r];
+        } else {
+            s_key[k] = kNoCell;
+        }
+    }
+    if constexpr (SUB) {
+        for (int k = t; k < RHX * RHY; k += 256) {
+            int lx = k % RHX, ly = k / RHX;
+            int c = (ly + 1) * RGX + lx + 1;
+            uint32_t key = cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
+            const int8_t* e = hull_table + (size_t)key * kHullBytes;
+#pragma unroll
+            for (int q = 0; q < 17; q++) s_hull[k * 17 + q] = e[q];
+        }
+        __syncthreads();
+        // smoothed polygons of the non-internal candidate cells
+        TjView tv = tj_view(im);
+        for (int k = t; k < RCX * RCY; k += 256) {
+            if (s_key[k] == kNoCell) continue;
+            int lx = k % RCX, ly = k / RCX;
+            int x = x0 - 1 + lx, y = y0 - 1 + ly;
+            uint32_t node = s_g[(ly + 2) * RGX + lx + 2];
+            if (node == 90) continue;  // internal node keeps its hull (kernel.cu:231)
+            int hk = (ly + 1) * RHX + lx + 1;
+            int n = s_hull[hk * 17];
+            int hx[2 * kMaxPts], hy[2 * kMaxPts];
+            for (int q = 0; q < n; q++) { hx[q] = s_hull[hk * 17 + 1 + q]; hy[q] = s_hull[hk * 17 + 9 + q]; }
+            auto nbr = [&](int e, int* lxs, int* lys) -> int {
+                int q = hk + edy(e) * RHX + edx(e);
+                int ln = s_hull[q * 17];
+                for (int u = 0; u < ln; u++) { lxs[u] = s_hull[q * 17 + 1 + u]; lys[u] = s_hull[q * 17 + 9 + u]; }
+                return ln;
+            };
+            auto tj = [&](int x4, int y4) -> bool { return tjunction(tv, x, y, x4, y4); };
+            int ox[kMaxPoly], oy[kMaxPoly];
+            int m = subdivide_cell(hx, hy, n, node, nbr, tj, ox, oy);
+            uint8_t tri[3 * kMaxPoly];
+            int ok;
+            int nt = ear_clip(ox, oy, m, tri, &ok);
+            for (int r = 0; r < MW; r++) s_mask[k * MW + r] = coverage_row(ox, oy, tri, nt, S, r);
+        }
+    }
+    __syncthreads();
+
+    // output: one unit = one output row of one cell (S pixels)
+    constexpr uint32_t FULL = (S >= 32) ? 0xFFFFFFFFu : ((1u << S) - 1u);
+    const int OH = it.H * S;
+    for (int u = t; u < RTX * RTY * S; u += 256) {
+        int cx = u % RTX, yr = u / RTX;
+        int cy = yr / S, ky = yr % S;
+        int gx = x0 + cx, gy = y0 + cy;
+        if (gx >= it.W || gy >= it.H) continue;
+        uint32_t col[S];
+#pragma unroll
+        for (int q = 0; q < S; q++) col[q] = 0xFF000000u;
+        uint32_t assigned = 0;
+#pragma unroll
+        for (int dy = 1; dy >= -1; dy--) {
+            int r = ky - dy * S - KMIN;
+            if (r < 0 || r >= MW) continue;
+#pragma unroll
+            for (int dx = 1; dx >= -1; dx--) {
+                int k = (cy + 1 + dy) * RCX + (cx + 1 + dx);
+                if (s_key[k] == kNoCell) continue;
+                uint32_t row = s_mask[k * MW + r];
+                int sh = -dx * S - KMIN;
+                uint32_t bits = (sh >= 0 ? (row >> sh) : (row << -sh)) & FULL;
+                uint32_t nb = bits & ~assigned;
+                if (nb) {
+                    uint32_t c = s_col[k];
+#pragma unroll
+                    for (int q = 0; q < S; q++)
+                        if ((nb >> q) & 1u) col[q] = c;
+                    assigned |= nb;
+                }
+            }
+        }
+        int Y = gy * S + ky;
+        int row = (it.flags & DPX_TOPDOWN) ? (OH - 1 - Y) : Y;
+        uint8_t* dst = out + it.out_off + (size_t)row * it.out_stride + (size_t)gx * S * 4;
+        if constexpr (S % 4 == 0) {
+            if (it.aligned16) {
+#pragma unroll
+                for (int q = 0; q < S; q += 4)
+                    *reinterpret_cast<uint4*>(dst + 4 * q) = make_uint4(col[q], col[q + 1], col[q + 2], col[q + 3]);
+                continue;
+            }
+        }
+        if constexpr (S % 2 == 0) {
+            if ((reinterpret_cast<uintptr_t>(dst) & 7) == 0) {
+#pragma unroll
+                for (int q = 0; q < S; q += 2)
+                    *reinterpret_cast<uint2*>(dst + 4 * q) = make_uint2(col[q], col[q + 1]);
+                continue;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < S; q++) *reinterpret_cast<uint32_t*>(dst + 4 * q) = col[q];
+    }
+}
+
+using RasterFn = void (*)(const DevItem*, const int32_t*, const uint32_t*, int, const uint8_t*, const uint8_t*,
+                          const int8_t*, const uint32_t*, uint8_t*);
+
+template <int S>
+RasterFn raster_fn(bool sub)
+{
+    return sub ? (RasterFn)kf_raster<S, true> : (RasterFn)kf_raster<S, false>;
+}
+
+RasterFn pick_raster(int S, bool sub)
+{
+    switch (S) {
+        case 1: return raster_fn<1>(sub);
+        case 2: return raster_fn<2>(sub);
+        case 3: return raster_fn<3>(sub);
+        case 4: return raster_fn<4>(sub);
+        case 5: return raster_fn<5>(sub);
+        case 6: return raster_fn<6>(sub);
+        case 7: return raster_fn<7>(sub);
+        case 8: return raster_fn<8>(sub);
+        case 9: return raster_fn<9>(sub);
+        case 10: return raster_fn<10>(sub);
+        case 11: return raster_fn<11>(sub);
+        case 12: return raster_fn<12>(sub);
+        case 13: return raster_fn<13>(sub);
+        case 14: return raster_fn<14>(sub);
+        case 15: return raster_fn<15>(sub);
+    }
+    return nullptr;
+}
+
+}  // namespace dpx
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct dpx_context {
+    int device = 0;
+    int8_t* d_hull = nullptr;
+    uint32_t* d_mask[dpx::kMaxScale + 1] = {};
+    uint8_t *d_gaux = nullptr, *d_gfin = nullptr;
+    size_t g_cap = 0;
+    uint8_t* d_desc = nullptr;
+    size_t desc_cap = 0;
+    uint8_t* h_pinned = nullptr;
+    size_t pinned_cap = 0;
+    hipEvent_t upload_done = nullptr;
+    bool upload_pending = false;
+    std::vector<uint8_t> last_desc;
+    std::vector<uint64_t> g_offsets;
+    std::mutex mu;
+    // kernel timing: 4 events per recorded batch (start, after graph, after heuristic, end)
+    bool timing = false;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used = 0;
+    dpx_batch_times acc{};
+};
+
+namespace {
+
+using namespace dpx;
+
+int ensure_mask(dpx_context* ctx, int S)
+{
+    if (ctx->d_mask[S]) return DPX_OK;
+    size_t n = (size_t)kKeys * mask_w(S);
+    DPX_HIP(hipMalloc(&ctx->d_mask[S], n * sizeof(uint32_t)));
+    hipLaunchKernelGGL(kt_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, ctx->d_hull, S, ctx->d_mask[S]);
+    DPX_HIP(hipGetLastError());
+    DPX_HIP(hipDeviceSynchronize());
+    return DPX_OK;
+}
+
+struct Group {
+    int scale;
+    bool sub;
+    std::vector<int32_t> idx;
+    std::vector<uint32_t> prefix;
+};
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+}  // namespace
+
+extern "C" int dpx_context_create(int device, dpx_context** out)
+{
+    DPX_CHECK(out, DPX_EINVAL, "null output pointer");
+    *out = nullptr;
+    if (int rc = require_device()) return rc;
+    DPX_HIP(hipSetDevice(device));
+    dpx_context* ctx = new dpx_context();
+    ctx->device = device;
+    auto cleanup = [&](int rc) {
+        dpx_context_destroy(ctx);
+        return rc;
+    };
+    if (hipMalloc(&ctx->d_hull, (size_t)kKeys * kHullBytes) != hipSuccess)
+        return cleanup(fail(DPX_ENOMEM, "table allocation failed"));
+    hipLaunchKernelGGL(kt_hull, dim3(kKeys / 256), dim3(256), 0, 0, ctx->d_hull);
+    if (hipGetLastError() != hipSuccess || hipDeviceSynchronize() != hipSuccess)
+        return cleanup(fail(DPX_EHIP, "cell table build failed"));
+    if (hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(DPX_EHIP, "event creation failed"));
+    *out = ctx;
+    return DPX_OK;
+}
+
+extern "C" void dpx_context_destroy(dpx_context* ctx)
+{
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(ctx->d_hull);
+    for (auto* m : ctx->d_mask) (void)hipFree(m);
+    (void)hipFree(ctx->d_gaux);
+    (void)hipFree(ctx->d_gfin);
+    (void)hipFree(ctx->d_desc);
+    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    if (ctx->upload_done) (void)hipEventDestroy(ctx->upload_done);
+    for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
+    delete ctx;
+}
+
+extern "C" int dpx_batch_graph(dpx_context* ctx, const uint8_t** d_graph, const uint64_t** offsets_host)
+{
+    DPX_CHECK(ctx && d_graph && offsets_host, DPX_EINVAL, "bad arguments");
+    *d_graph = ctx->d_gfin;
+    *offsets_host = ctx->g_offsets.data();
+    return DPX_OK;
+}
+
+extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_out, const dpx_item* items,
+                             int n_items, void* stream_)
+{
+    DPX_CHECK(ctx && d_in && d_out && items && n_items > 0, DPX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t stream = as_stream(stream_);
+    DPX_HIP(hipSetDevice(ctx->device));
+
+    // plan: device items, graph tiles, raster groups by (scale, subdivide)
+    std::vector<DevItem> dev(n_items);
+    std::vector<uint32_t> gprefix(n_items + 1, 0);
+    ctx->g_offsets.assign(n_items, 0);
+    std::map<int, Group> groups;
+    uint64_t goff = 0;
+    for (int k = 0; k < n_items; k++) {
+        const dpx_item& a = items[k];
+        DPX_CHECK(a.width > 0 && a.height > 0 && a.width <= 32768 && a.height <= 32768, DPX_EINVAL,
+                  "item %d: bad size %dx%d", k, a.width, a.height);
+        DPX_CHECK(a.channels == 3 || a.channels == 4, DPX_EINVAL, "item %d: channels %d", k, a.channels);
+        DPX_CHECK(a.stride >= a.width * a.channels, DPX_EINVAL, "item %d: stride %d", k, a.stride);
+        DPX_CHECK(a.scale >= 1 && a.scale <= kMaxScale, DPX_EINVAL, "item %d: scale %d outside 1..%d", k, a.scale,
+                  kMaxScale);
+        DPX_CHECK(a.out_stride >= 4 * a.width * a.scale && a.out_stride % 4 == 0 && a.out_offset % 4 == 0, DPX_EINVAL,
+                  "item %d: output layout (stride %d, offset %llu)", k, a.out_stride,
+                  (unsigned long long)a.out_offset);
+        DevItem& d = dev[k];
+        memset(&d, 0, sizeof(d));
+        d.in_off = a.in_offset;
+        d.out_off = a.out_offset;
+        d.g_off = goff;
+        d.W = a.width;
+        d.H = a.height;
+        d.stride = a.stride;
+        d.ch = a.channels;
+        d.flags = a.flags;
+        d.out_stride = a.out_stride;
+        d.scale = a.scale;
+        d.subdivide = a.subdivide ? 1 : 0;
+        d.aligned16 = (a.out_stride % 16 == 0) && (a.out_offset % 16 == 0) &&
+                      (reinterpret_cast<uintptr_t>(d_out) % 16 == 0);
+        ctx->g_offsets[k] = goff;
+        goff += align_up((size_t)a.width * a.height, 256);
+        uint32_t gt = (uint32_t)(((a.width + GTX - 1) / GTX) * ((a.height + GTY - 1) / GTY));
+        gprefix[k + 1] = gprefix[k] + gt;
+        int gkey = a.scale * 2 + d.subdivide;
+        Group& grp = groups[gkey];
+        grp.scale = a.scale;
+        grp.sub = d.subdivide;
+        if (grp.prefix.empty()) grp.prefix.push_back(0);
+        grp.idx.push_back(k);
+        uint32_t rt = (uint32_t)(((a.width + RTX - 1) / RTX) * ((a.height + RTY - 1) / RTY));
+        grp.prefix.push_back(grp.prefix.back() + rt);
+    }
+    for (auto& kv : groups)
+        if (int rc = ensure_mask(ctx, kv.second.scale)) return rc;
+
+    // graph arenas (grown only when a batch needs more)
+    if (goff > ctx->g_cap) {
+        DPX_HIP(hipDeviceSynchronize());
+        (void)hipFree(ctx->d_gaux);
+        (void)hipFree(ctx->d_gfin);
+        ctx->d_gaux = ctx->d_gfin = nullptr;
+        ctx->g_cap = 0;
+        size_t cap = align_up(goff + goff / 4, 1 << 20);
+        DPX_HIP(hipMalloc(&ctx->d_gaux, cap));
+        DPX_HIP(hipMalloc(&ctx->d_gfin, cap));
+        ctx->g_cap = cap;
+    }
+
+    // descriptor blob: items | gprefix | per group (idx, prefix)
+    std::vector<uint8_t> blob;
+    auto put = [&](const void* p, size_t n) {
+        size_t off = align_up(blob.size(), 64);
+        blob.resize(off + n);
+        memcpy(blob.data() + off, p, n);
+        return off;
+    };
+    size_t off_items = put(dev.data(), dev.size() * sizeof(DevItem));
+    size_t off_gprefix = put(gprefix.data(), gprefix.size() * sizeof(uint32_t));
+    std::vector<std::pair<size_t, size_t>> goffs;
+    for (auto& kv : groups) {
+        size_t a = put(kv.second.idx.data(), kv.second.idx.size() * sizeof(int32_t));
+        size_t b = put(kv.second.prefix.data(), kv.second.prefix.size() * sizeof(uint32_t));
+        goffs.push_back({ a, b });
+    }
+    if (blob != ctx->last_desc) {
+        if (ctx->upload_pending) {
+            DPX_HIP(hipEventSynchronize(ctx->upload_done));
+            ctx->upload_pending = false;
+        }
+        if (blob.size() > ctx->desc_cap) {
+            DPX_HIP(hipDeviceSynchronize());
+            (void)hipFree(ctx->d_desc);
+            ctx->d_desc = nullptr;
+            if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+            ctx->h_pinned = nullptr;
+            size_t cap = align_up(blob.size() * 2, 1 << 16);
+            DPX_HIP(hipMalloc(&ctx->d_desc, cap));
+            DPX_HIP(hipHostMalloc(&ctx->h_pinned, cap, hipHostMallocDefault));
+            ctx->desc_cap = cap;
+            ctx->pinned_cap = cap;
+        }
+        memcpy(ctx->h_pinned, blob.data(), blob.size());
+        DPX_HIP(hipMemcpyAsync(ctx->d_desc, ctx->h_pinned, blob.size(), hipMemcpyHostToDevice, stream));
+        DPX_HIP(hipEventRecord(ctx->upload_done, stream));
+        ctx->upload_pending = true;
+        ctx->last_desc.swap(blob);
+    }
+    const DevItem* d_items = reinterpret_cast<const DevItem*>(ctx->d_desc + off_items);
+    const uint32_t* d_gprefix = reinterpret_cast<const uint32_t*>(ctx->d_desc + off_gprefix);
+
+    hipEvent_t ev[4] = {};
+    if (ctx->timing) {
+        while (ctx->ev_pool.size() < ctx->ev_used + 4) {
+            hipEvent_t e;
+            DPX_HIP(hipEventCreate(&e));
+            ctx->ev_pool.push_back(e);
+        }
+        for (int q = 0; q < 4; q++) ev[q] = ctx->ev_pool[ctx->ev_used + q];
+        ctx->ev_used += 4;
+        DPX_HIP(hipEventRecord(ev[0], stream));
+    }
+    hipLaunchKernelGGL(kf_graph, dim3(gprefix[n_items]), dim3(256), 0, stream, d_items, d_gprefix, n_items, d_in,
+                       ctx->d_gaux);
+    if (ctx->timing) DPX_HIP(hipEventRecord(ev[1], stream));
+    hipLaunchKernelGGL(kf_heur, dim3(gprefix[n_items]), dim3(256), 0, stream, d_items, d_gprefix, n_items,
+                       ctx->d_gaux, ctx->d_gfin);
+    if (ctx->timing) DPX_HIP(hipEventRecord(ev[2], stream));
+    int gk = 0;
+    for (auto& kv : groups) {
+        Group& grp = kv.second;
+        RasterFn fn = pick_raster(grp.scale, grp.sub);
+        DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", grp.scale);
+        const int32_t* d_idx = reinterpret_cast<const int32_t*>(ctx->d_desc + goffs[gk].first);
+        const uint32_t* d_rp = reinterpret_cast<const uint32_t*>(ctx->d_desc + goffs[gk].second);
+        hipLaunchKernelGGL(fn, dim3(grp.prefix.back()), dim3(256), 0, stream, d_items, d_idx, d_rp,
+                           (int)grp.idx.size(), d_in, ctx->d_gfin, ctx->d_hull, ctx->d_mask[grp.scale], d_out);
+        gk++;
+    }
+    if (ctx->timing) DPX_HIP(hipEventRecord(ev[3], stream));
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_batch_timing(dpx_context* ctx, int enable)
+{
+    DPX_CHECK(ctx, DPX_EINVAL, "null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->timing = enable != 0;
+    return DPX_OK;
+}
+
+extern "C" int dpx_batch_timing_read(dpx_context* ctx, dpx_batch_times* out)
+{
+    DPX_CHECK(ctx && out, DPX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    for (size_t b = 0; b + 4 <= ctx->ev_used; b += 4) {
+        hipEvent_t* ev = &ctx->ev_pool[b];
+        DPX_HIP(hipEventSynchronize(ev[3]));
+        float t01, t12, t23;
+        DPX_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
+        DPX_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
+        DPX_HIP(hipEventElapsedTime(&t23, ev[2], ev[3]));
+        ctx->acc.graph_ms += t01;
+        ctx->acc.heuristic_ms += t12;
+        ctx->acc.raster_ms += t23;
+        ctx->acc.batches++;
+    }
+    ctx->ev_used = 0;
+    *out = ctx->acc;
+    ctx->acc = dpx_batch_times{};
+    return DPX_OK;
+}

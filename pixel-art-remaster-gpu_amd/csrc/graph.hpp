@@ -1,0 +1,111 @@
+// graph.hpp -- similarity-graph crossing resolution (trivial and heuristic).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "geom.hpp"
+
+namespace dpx {
+
+// Number of set bits of `node` other than `edge`, and the lowest of them.
+// The reference counts them with an 8-step loop (graph_functions.cu:427-441,
+// 462-469, 548-555); popcount gives the same number.
+__host__ __device__ inline int others(uint32_t node, int edge)
+{
+    return __builtin_popcount(node & 0xFFu & ~(1u << edge));
+}
+
+// Offset of the neighbour across edge e in a row-major W-wide graph
+// (calc_index, graph_functions.cu:46-76).
+__host__ __device__ inline int nbr_index(int index, int e, int W) { return index + edy(e) * W + edx(e); }
+
+// calcVal2PathSize (graph_functions.cu:541-573) as a loop: follow valence-2
+// nodes away from `edge`, adding one per step to the shared accumulator
+// while it is <= 30 (so a diagonal's two walks sum to at most 31).
+__host__ __device__ inline void val2_walk(const uint8_t* g, int index, int edge, int* result, int W)
+{
+    for (;;) {
+        uint32_t rest = (uint32_t)g[index] & ~(1u << edge);
+        if (*result > 30) return;
+        if (__builtin_popcount(rest) != 1) return;
+        (*result)++;
+        int k = __builtin_ctz(rest);
+        index = nbr_index(index, k, W);
+        edge = 7 - k;
+    }
+}
+
+// processHeuristics2 (graph_functions.cu:761-849) for the 2x2 block whose
+// lower-left pixel is (i,j): i1=(i,j) i2=(i,j+1) i3=(i+1,j) i4=(i+1,j+1).
+// true: remove "/" (i1-i4, bits 2 and 5); false: remove "\" (i2-i3, bits 0 and 7).
+__host__ __device__ inline bool remove_slash(const uint8_t* g, int W, int i, int j)
+{
+    int i1 = j * W + i, i2 = i1 + W, i3 = i1 + 1, i4 = i1 + W + 1;
+    uint32_t n1 = g[i1], n2 = g[i2], n3 = g[i3], n4 = g[i4];
+    int o1 = others(n1, 2), o4 = others(n4, 5), o3 = others(n3, 0), o2 = others(n2, 7);
+    // curves (checkValence2Edge 421-453)
+    if (o1 == 1 && o4 == 1) return false;
+    if (o3 == 1 && o2 == 1) return true;
+    // islands (checkValence2Vertex 458-479), with the precedence of 805-806
+    bool v1 = o1 == 0, v4 = o4 == 0, v3 = o3 == 0, v2 = o2 == 0;
+    if ((v1 || v4) && !v3 && !v2) return false;
+    if (v3 || (v2 && !v1 && !v4)) return true;
+    // curve lengths; ties remove "/"
+    int seg = 0, seg2 = 0;
+    val2_walk(g, i1, 2, &seg, W);
+    val2_walk(g, i4, 5, &seg, W);
+    val2_walk(g, i3, 0, &seg2, W);
+    val2_walk(g, i2, 7, &seg2, W);
+    return !(seg2 < seg);
+}
+
+// Diagonal bits (0,2,5,7) the pixel at (i,j) keeps after crossCheck_4
+// (graph_functions.cu:1211-1274), given its byte and the bytes of its 8
+// neighbours (nb[e] = byte of neighbour e, 0 off-frame).  A 2x2 block whose
+// four orthogonal edges are all present loses both diagonals.
+__host__ __device__ inline uint32_t trivial_clear_mask(uint32_t c, const uint32_t* nb)
+{
+    uint32_t clear = 0;
+    // up-left block: pixels left(3), up-left(0), self, up(1)
+    if (bit(nb[3], 1) && bit(nb[3], 4) && bit(nb[0], 6) && bit(nb[0], 4) && bit(c, 3) && bit(c, 1) &&
+        bit(nb[1], 6) && bit(nb[1], 3))
+        clear |= 1u << 0;
+    // up-right block: self, up(1), right(4), up-right(2)
+    if (bit(c, 1) && bit(c, 4) && bit(nb[1], 6) && bit(nb[1], 4) && bit(nb[4], 3) && bit(nb[4], 1) &&
+        bit(nb[2], 6) && bit(nb[2], 3))
+        clear |= 1u << 2;
+    // down-left block: down-left(5), left(3), down(6), self
+    if (bit(nb[5], 1) && bit(nb[5], 4) && bit(nb[3], 6) && bit(nb[3], 4) && bit(nb[6], 3) && bit(nb[6], 1) &&
+        bit(c, 6) && bit(c, 3))
+        clear |= 1u << 5;
+    // down-right block: down(6), self, down-right(7), right(4)
+    if (bit(nb[6], 1) && bit(nb[6], 4) && bit(c, 6) && bit(c, 4) && bit(nb[7], 3) && bit(nb[7], 1) &&
+        bit(nb[4], 6) && bit(nb[4], 3))
+        clear |= 1u << 7;
+    return clear;
+}
+
+// Diagonal bits the pixel (i,j) loses in ambiguous_cross_Kernel
+// (crossCheck_Heuristics graph_functions.cu:1433-1518), reading the snapshot
+// `aux`.  nb[e] = snapshot byte of neighbour e (0 off-frame); c = own byte.
+__host__ __device__ inline uint32_t heuristic_clear_mask(const uint8_t* aux, int W, int i, int j, uint32_t c,
+                                                         const uint32_t* nb)
+{
+    uint32_t clear = 0;
+    // up-left block, origin (i-1,j): crossed if left has bit 2, up-left bit 7, self bit 0, up bit 5
+    if (bit(nb[3], 2) && bit(nb[0], 7) && bit(c, 0) && bit(nb[1], 5))
+        if (!remove_slash(aux, W, i - 1, j)) clear |= 1u << 0;
+    // down-left block, origin (i-1,j-1)
+    if (bit(nb[5], 2) && bit(nb[3], 7) && bit(nb[6], 0) && bit(c, 5))
+        if (remove_slash(aux, W, i - 1, j - 1)) clear |= 1u << 5;
+    // down-right block, origin (i,j-1)
+    if (bit(nb[6], 2) && bit(c, 7) && bit(nb[7], 0) && bit(nb[4], 5))
+        if (!remove_slash(aux, W, i, j - 1)) clear |= 1u << 7;
+    // up-right block, origin (i,j)
+    if (bit(c, 2) && bit(nb[1], 7) && bit(nb[4], 0) && bit(nb[2], 5))
+        if (remove_slash(aux, W, i, j)) clear |= 1u << 2;
+    return clear;
+}
+
+}  // namespace dpx

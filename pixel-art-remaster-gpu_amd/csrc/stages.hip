@@ -1,0 +1,561 @@
+// stages.hip -- the per-stage kernels (one kernel per stage of kernel.cu's
+// sequence, intermediates in HBM in the reference's formats), the stage entry
+// points of the C ABI, the plain per-frame path and the launch_kernel drop-in.
+//
+// Launch shape: 16x16 threads, one thread per pixel (or per output pixel for
+// the raster).  These kernels favour a direct mapping of the reference's data
+// formats over speed; the batched path in fused.hip is the fast one.
+#include <stdlib.h>
+#include <string.h>
+
+#include "common.hpp"
+#include "geom.hpp"
+#include "graph.hpp"
+#include "pixel.hpp"
+
+namespace dpx {
+namespace {
+
+constexpr int kB = 16;
+
+dim3 grid2(int W, int H) { return dim3((unsigned)((W + kB - 1) / kB), (unsigned)((H + kB - 1) / kB)); }
+
+__device__ inline bool in_frame(int i, int j, int W, int H) { return i >= 0 && j >= 0 && i < W && j < H; }
+
+// graph_Kernel (kernel.cu:140-159): 8 similarity bits per pixel
+__global__ void k_graph(ImgView im, uint8_t* __restrict__ graph)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= im.W || j >= im.H) return;
+    Yuv c = yuv_of(load_px(im, i, j), im.flags, im.ch);
+    uint32_t g = 0;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        int ni = i + edx(e), nj = j + edy(e);
+        if (!in_frame(ni, nj, im.W, im.H)) continue;
+        if (similar(c, yuv_of(load_px(im, ni, nj), im.flags, im.ch))) g |= 1u << e;
+    }
+    graph[(size_t)j * im.W + i] = (uint8_t)g;
+}
+
+__device__ inline void gather_nb(const uint8_t* g, int W, int H, int i, int j, uint32_t* nb)
+{
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+        int ni = i + edx(e), nj = j + edy(e);
+        nb[e] = in_frame(ni, nj, W, H) ? (uint32_t)g[(size_t)nj * W + ni] : 0u;
+    }
+}
+
+// trivial_cross_Kernel (kernel.cu:162-177), in place: reads orthogonal bits,
+// clears the pixel's own diagonal bits only.
+__global__ void k_trivial(uint8_t* graph, int W, int H)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= W || j >= H) return;
+    uint32_t nb[8];
+    gather_nb(graph, W, H, i, j, nb);
+    uint8_t* p = &graph[(size_t)j * W + i];
+    uint32_t c = *p;
+    *p = (uint8_t)(c & ~trivial_clear_mask(c, nb));
+}
+
+// ambiguous_cross_Kernel (kernel.cu:180-189): decisions on the snapshot aux
+__global__ void k_heuristic(uint8_t* __restrict__ graph, const uint8_t* __restrict__ aux, int W, int H)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= W || j >= H) return;
+    uint32_t nb[8];
+    gather_nb(aux, W, H, i, j, nb);
+    size_t idx = (size_t)j * W + i;
+    uint32_t c = aux[idx];
+    graph[idx] = (uint8_t)(c & ~heuristic_clear_mask(aux, W, i, j, c, nb));
+}
+
+// cells_Kernel (kernel.cu:192-213): hull points in 45-slot cells.  Slots past
+// the closing point are written as (0,0).
+__global__ void k_cells(const uint8_t* __restrict__ graph, int W, int H, dpx_point* __restrict__ diagram,
+                        int32_t* __restrict__ edge_count)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= W || j >= H) return;
+    size_t N = (size_t)W * H, idx = (size_t)j * W + i;
+    uint32_t left = idx > 0 ? graph[idx - 1] : 0u;
+    uint32_t right = idx + 1 < N ? graph[idx + 1] : 0u;
+    int hx[2 * kMaxPts], hy[2 * kMaxPts];
+    int n = cell_hull(cell_key(graph[idx], left, right), hx, hy);
+    dpx_point* c = &diagram[idx * kCellSize];
+    for (int t = 0; t < kCellSize; t++) {
+        dpx_point p;
+        p.x = t <= n ? 0.25f * (float)hx[t] : 0.0f;
+        p.y = t <= n ? 0.25f * (float)hy[t] : 0.0f;
+        c[t] = p;
+    }
+    edge_count[idx] = n;
+}
+
+__device__ inline int load_quarter(const dpx_point* c, int n, int* x, int* y)
+{
+    for (int t = 0; t < n; t++) {
+        x[t] = (int)(c[t].x * 4.0f);
+        y[t] = (int)(c[t].y * 4.0f);
+    }
+    return n;
+}
+
+// subdivision_Kernel (kernel.cu:216-261) on the snapshots daux/eaux.
+__global__ void k_subdivide(ImgView im, const uint8_t* __restrict__ graph, dpx_point* __restrict__ diagram,
+                            const dpx_point* __restrict__ daux, int32_t* __restrict__ edge_count,
+                            const int32_t* __restrict__ eaux)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    int W = im.W;
+    if (i >= W || j >= im.H) return;
+    int idx = j * W + i;
+    uint32_t node = graph[idx];
+    if (node == 90) return;  // internal node (kernel.cu:231)
+    int n = eaux[idx];
+    if (n < 3 || n > 2 * kMaxPts - 1) return;
+    int hx[2 * kMaxPts], hy[2 * kMaxPts];
+    load_quarter(&daux[(size_t)idx * kCellSize], n, hx, hy);
+    TjView tv = tj_view(im);
+    auto nbr = [&](int e, int* lx, int* ly) -> int {
+        int q = idx + edy(e) * W + edx(e);
+        int ln = eaux[q];
+        if (ln > 2 * kMaxPts - 1) ln = 2 * kMaxPts - 1;
+        return load_quarter(&daux[(size_t)q * kCellSize], ln, lx, ly);
+    };
+    auto tj = [&](int x4, int y4) -> bool { return tjunction(tv, i, j, x4, y4); };
+    int ox[kMaxPoly], oy[kMaxPoly];
+    int m = subdivide_cell(hx, hy, n, node, nbr, tj, ox, oy);
+    dpx_point* c = &diagram[(size_t)idx * kCellSize];
+    for (int t = 0; t < m; t++) {
+        dpx_point p;
+        p.x = (float)ox[t] * (1.0f / 64.0f);
+        p.y = (float)oy[t] * (1.0f / 64.0f);
+        c[t] = p;
+    }
+    edge_count[idx] = m;
+}
+
+// triangulate_Kernel (kernel.cu:264-282): ear clipping, then every slot gets
+// the pixel offset; unwritten slots stay at the pixel origin.
+__global__ void k_triangulate(dpx_point* __restrict__ diagram, const int32_t* __restrict__ edge_count, int W,
+                              int H)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= W || j >= H) return;
+    size_t idx = (size_t)j * W + i;
+    dpx_point* c = &diagram[idx * kCellSize];
+    int n = edge_count[idx];
+    int x[kMaxPoly], y[kMaxPoly];
+    uint8_t tri[3 * kMaxPoly];
+    int nt = 0, ok = 0;
+    if (n >= 3 && n <= kMaxPoly) {
+        for (int t = 0; t < n; t++) {
+            x[t] = (int)(c[t].x * 64.0f);
+            y[t] = (int)(c[t].y * 64.0f);
+        }
+        nt = ear_clip(x, y, n, tri, &ok);
+    }
+    float fi = (float)i, fj = (float)j;
+    for (int t = 0; t < kCellSize; t++) {
+        dpx_point p;
+        if (t < 3 * nt) {
+            p.x = (float)x[tri[t]] * (1.0f / 64.0f) + fi;
+            p.y = (float)y[tri[t]] * (1.0f / 64.0f) + fj;
+        } else {
+            p.x = fi;
+            p.y = fj;
+        }
+        c[t] = p;
+    }
+}
+
+// color_kernel (kernel.cu:67-104) + position_kernel (kernel.cu:109-137)
+__global__ void k_export_vbo(ImgView im, const dpx_point* __restrict__ diagram, const int32_t* __restrict__ edge_count,
+                             dpx_point* __restrict__ pos, dpx_uchar4* __restrict__ color)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= im.W || j >= im.H) return;
+    size_t idx = (size_t)j * im.W + i;
+    uint32_t rgba = cell_rgba(load_px(im, i, j), im.ch);
+    dpx_uchar4 c4;
+    c4.x = (uint8_t)(rgba & 0xFF);
+    c4.y = (uint8_t)((rgba >> 8) & 0xFF);
+    c4.z = (uint8_t)((rgba >> 16) & 0xFF);
+    c4.w = (uint8_t)(rgba >> 24);
+    int valid = (edge_count[idx] - 2) * 3;
+    for (int t = 0; t < kCellSize; t++) {
+        size_t s = idx * kCellSize + t;
+        if (color) color[s] = c4;
+        if (pos) {
+            dpx_point p;
+            if (t < valid) p = diagram[s];
+            else { p.x = -100.0f; p.y = -100.0f; }
+            pos[s] = p;
+        }
+    }
+}
+
+// glDrawArrays(GL_TRIANGLES) in VBO order (simpleVBO.cpp:281) as a gather:
+// each output pixel takes the colour of the highest-index cell (the last one
+// drawn) with a triangle covering its centre.  Cells reach at most 0.34 past
+// their unit square, so the 3x3 cells around the pixel are the candidates.
+__global__ void k_raster_tri(ImgView im, const dpx_point* __restrict__ diagram, const int32_t* __restrict__ edge_count,
+                             int scale, uint8_t* __restrict__ out, int out_stride)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    int Y = blockIdx.y * blockDim.y + threadIdx.y;
+    int OW = im.W * scale, OH = im.H * scale;
+    if (X >= OW || Y >= OH) return;
+    int cx = X / scale, cy = Y / scale;
+    uint32_t color = 0xFF000000u;  // clear colour (0,0,0,1), main.cpp:260
+    bool done = false;
+    for (int dy = 1; dy >= -1 && !done; dy--)
+        for (int dx = 1; dx >= -1 && !done; dx--) {
+            int ci = cx + dx, cj = cy + dy;
+            if (!in_frame(ci, cj, im.W, im.H)) continue;
+            size_t k = (size_t)cj * im.W + ci;
+            int nt = edge_count[k] - 2;
+            if (nt > kCellSize / 3) nt = kCellSize / 3;
+            int sx = (2 * X + 1) * 64 - ci * 128 * scale, sy = (2 * Y + 1) * 64 - cj * 128 * scale;
+            const dpx_point* c = &diagram[k * kCellSize];
+            float fi = (float)ci, fj = (float)cj;
+            for (int t = 0; t < nt; t++) {
+                int v[6];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    dpx_point p = c[3 * t + q];
+                    v[2 * q] = (int)((p.x - fi) * 64.0f) * 2 * scale;
+                    v[2 * q + 1] = (int)((p.y - fj) * 64.0f) * 2 * scale;
+                }
+                if (tri_covers(v[0], v[1], v[2], v[3], v[4], v[5], sx, sy)) {
+                    color = cell_rgba(load_px(im, ci, cj), im.ch);
+                    done = true;
+                    break;
+                }
+            }
+        }
+    int row = (im.flags & DPX_TOPDOWN) ? (OH - 1 - Y) : Y;
+    *reinterpret_cast<uint32_t*>(out + (size_t)row * out_stride + (size_t)X * 4) = color;
+}
+
+// ---------------------------------------------------------------------------
+// Components: lock-free union-find over the graph's edges.  Roots are hooked
+// under smaller roots only, so each component's root is its smallest index.
+// ---------------------------------------------------------------------------
+__device__ inline int uf_find(int32_t* parent_, int v)
+{
+    // other threads hook roots and halve paths concurrently: read through volatile
+    volatile int32_t* parent = parent_;
+    int p = parent[v];
+    while (p != v) {
+        int gp = parent[p];
+        if (gp != p) parent[v] = gp;  // path halving; any ancestor is a valid parent
+        v = p;
+        p = gp;
+    }
+    return v;
+}
+
+__global__ void k_cc_init(int32_t* parent, int N)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N) parent[k] = k;
+}
+
+__global__ void k_cc_union(const uint8_t* __restrict__ graph, int W, int H, int32_t* parent)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    if (i >= W || j >= H) return;
+    int v = j * W + i;
+    uint32_t g = graph[v];
+    // edges towards smaller indices only (each edge once): bits 3, 5, 6, 7
+    const int es[4] = { 3, 5, 6, 7 };
+    for (int t = 0; t < 4; t++) {
+        int e = es[t];
+        if (!bit(g, e)) continue;
+        int ni = i + edx(e), nj = j + edy(e);
+        if (!in_frame(ni, nj, W, H)) continue;
+        int u = nj * W + ni;
+        for (;;) {
+            int ru = uf_find(parent, u), rv = uf_find(parent, v);
+            if (ru == rv) break;
+            int hi = ru > rv ? ru : rv, lo = ru > rv ? rv : ru;
+            if (atomicCAS(&parent[hi], hi, lo) == hi) break;
+        }
+    }
+}
+
+__global__ void k_cc_final(int32_t* parent, int32_t* labels, int N)
+{
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < N) labels[k] = uf_find(parent, k);
+}
+
+int check_img(const void* p, int W, int H, int stride, int ch)
+{
+    DPX_CHECK(p != nullptr, DPX_EINVAL, "null image pointer");
+    DPX_CHECK(W > 0 && H > 0 && W <= 32768 && H <= 32768, DPX_EINVAL, "bad frame size %dx%d", W, H);
+    DPX_CHECK(ch == 3 || ch == 4, DPX_EINVAL, "channels must be 3 or 4, got %d", ch);
+    DPX_CHECK(stride >= W * ch, DPX_EINVAL, "stride %d < width*channels %d", stride, W * ch);
+    return DPX_OK;
+}
+
+}  // namespace
+
+// -------------------------------------------------------------------------
+// stage entry points
+// -------------------------------------------------------------------------
+extern "C" int dpx_stage_similarity_graph(const uint8_t* d_img, int W, int H, int stride, int ch, uint32_t flags,
+                                          uint8_t* d_graph, void* stream)
+{
+    if (int rc = check_img(d_img, W, H, stride, ch)) return rc;
+    DPX_CHECK(d_graph, DPX_EINVAL, "null graph pointer");
+    ImgView im{ d_img, W, H, stride, ch, flags };
+    hipLaunchKernelGGL(k_graph, grid2(W, H), dim3(kB, kB), 0, as_stream(stream), im, d_graph);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_stage_trivial_cross(uint8_t* d_graph, int W, int H, void* stream)
+{
+    DPX_CHECK(d_graph && W > 0 && H > 0, DPX_EINVAL, "bad arguments");
+    hipLaunchKernelGGL(k_trivial, grid2(W, H), dim3(kB, kB), 0, as_stream(stream), d_graph, W, H);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_stage_heuristic_cross(uint8_t* d_graph, uint8_t* d_graph_aux, int W, int H, void* stream)
+{
+    DPX_CHECK(d_graph && d_graph_aux && W > 0 && H > 0, DPX_EINVAL, "bad arguments");
+    DPX_HIP(hipMemcpyAsync(d_graph_aux, d_graph, (size_t)W * H, hipMemcpyDeviceToDevice, as_stream(stream)));
+    hipLaunchKernelGGL(k_heuristic, grid2(W, H), dim3(kB, kB), 0, as_stream(stream), d_graph, d_graph_aux, W, H);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_stage_cells(const uint8_t* d_graph, int W, int H, dpx_point* d_diagram, int32_t* d_edge_count,
+                               void* stream)
+{
+    DPX_CHECK(d_graph && d_diagram && d_edge_count && W > 0 && H > 0, DPX_EINVAL, "bad arguments");
+    hipLaunchKernelGGL(k_cells, grid2(W, H), dim3(kB, kB), 0, as_stream(stream), d_graph, W, H, d_diagram,
+                       d_edge_count);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_stage_subdivide(const uint8_t* d_img, int W, int H, int stride, int ch, uint32_t flags,
+                                   const uint8_t* d_graph, dpx_point* d_diagram, dpx_point* d_diagram_aux,
+                                   int32_t* d_edge_count, int32_t* d_edge_count_aux, void* stream)
+{
+    if (int rc = check_img(d_img, W, H, stride, ch)) return rc;
+    DPX_CHECK(d_graph && d_diagram && d_diagram_aux && d_edge_count && d_edge_count_aux, DPX_EINVAL,
+              "bad arguments");
+    size_t N = (size_t)W * H;
+    hipStream_t s = as_stream(stream);
+    DPX_HIP(hipMemcpyAsync(d_diagram_aux, d_diagram, N * kCellSize * sizeof(dpx_point), hipMemcpyDeviceToDevice, s));
+    DPX_HIP(hipMemcpyAsync(d_edge_count_aux, d_edge_count, N * sizeof(int32_t), hipMemcpyDeviceToDevice, s));
+    ImgView im{ d_img, W, H, stride, ch, flags };
+    hipLaunchKernelGGL(k_subdivide, grid2(W, H), dim3(kB, kB), 0, s, im, d_graph, d_diagram, d_diagram_aux,
+                       d_edge_count, d_edge_count_aux);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_stage_triangulate(dpx_point* d_diagram, const int32_t* d_edge_count, int W, int H, void* stream)
+{
+    DPX_CHECK(d_diagram && d_edge_count && W > 0 && H > 0, DPX_EINVAL, "bad arguments");
+    hipLaunchKernelGGL(k_triangulate, grid2(W, H), dim3(kB, kB), 0, as_stream(stream), d_diagram, d_edge_count, W, H);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_stage_export_vbo(const uint8_t* d_img, int W, int H, int stride, int ch, uint32_t flags,
+                                    const dpx_point* d_diagram, const int32_t* d_edge_count, dpx_point* d_pos,
+                                    dpx_uchar4* d_color, void* stream)
+{
+    if (int rc = check_img(d_img, W, H, stride, ch)) return rc;
+    DPX_CHECK(d_diagram && d_edge_count && (d_pos || d_color), DPX_EINVAL, "bad arguments");
+    ImgView im{ d_img, W, H, stride, ch, flags };
+    hipLaunchKernelGGL(k_export_vbo, grid2(W, H), dim3(kB, kB), 0, as_stream(stream), im, d_diagram, d_edge_count,
+                       d_pos, d_color);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_stage_rasterize(const uint8_t* d_img, int W, int H, int stride, int ch, uint32_t flags,
+                                   const dpx_point* d_diagram, const int32_t* d_edge_count, int scale,
+                                   uint8_t* d_out, int out_stride, void* stream)
+{
+    if (int rc = check_img(d_img, W, H, stride, ch)) return rc;
+    DPX_CHECK(d_diagram && d_edge_count && d_out, DPX_EINVAL, "bad arguments");
+    DPX_CHECK(scale >= 1 && scale <= 32, DPX_EINVAL, "scale %d outside 1..32", scale);
+    DPX_CHECK((long long)W * scale <= 65536 && (long long)H * scale <= 65536, DPX_EINVAL, "output too large");
+    DPX_CHECK(out_stride >= 4 * W * scale && out_stride % 4 == 0, DPX_EINVAL, "bad out_stride %d", out_stride);
+    ImgView im{ d_img, W, H, stride, ch, flags };
+    hipLaunchKernelGGL(k_raster_tri, grid2(W * scale, H * scale), dim3(kB, kB), 0, as_stream(stream), im, d_diagram,
+                       d_edge_count, scale, d_out, out_stride);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_stage_components(const uint8_t* d_graph, int W, int H, int32_t* d_labels, void* stream)
+{
+    DPX_CHECK(d_graph && d_labels && W > 0 && H > 0, DPX_EINVAL, "bad arguments");
+    int N = W * H;
+    hipStream_t s = as_stream(stream);
+    // d_labels doubles as the parent array, then receives the roots
+    hipLaunchKernelGGL(k_cc_init, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, d_labels, N);
+    hipLaunchKernelGGL(k_cc_union, grid2(W, H), dim3(kB, kB), 0, s, d_graph, W, H, d_labels);
+    hipLaunchKernelGGL(k_cc_final, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, d_labels, d_labels, N);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+}  // namespace dpx
+
+// -------------------------------------------------------------------------
+// Plain per-frame path: kernel.cu's sequence with per-call allocation,
+// blocking copies and one kernel per stage, plus the raster.
+// -------------------------------------------------------------------------
+namespace {
+
+struct FrameBuffers {
+    uint8_t *img = nullptr, *graph = nullptr, *graph_aux = nullptr, *out = nullptr;
+    dpx_point *diagram = nullptr, *diagram_aux = nullptr, *pos = nullptr;
+    int32_t *ec = nullptr, *ec_aux = nullptr;
+    ~FrameBuffers()
+    {
+        (void)hipFree(img);
+        (void)hipFree(graph);
+        (void)hipFree(graph_aux);
+        (void)hipFree(out);
+        (void)hipFree(diagram);
+        (void)hipFree(diagram_aux);
+        (void)hipFree(ec);
+        (void)hipFree(ec_aux);
+    }
+};
+
+// Runs the stages up to triangulation on a frame already in fb.img.
+int run_stages(FrameBuffers& fb, int W, int H, int stride, int ch, uint32_t flags, bool subdivide)
+{
+    using namespace dpx;
+    size_t N = (size_t)W * H;
+    if (int rc = dpx_stage_similarity_graph(fb.img, W, H, stride, ch, flags, fb.graph, nullptr)) return rc;
+    if (int rc = dpx_stage_trivial_cross(fb.graph, W, H, nullptr)) return rc;
+    if (int rc = dpx_stage_heuristic_cross(fb.graph, fb.graph_aux, W, H, nullptr)) return rc;
+    if (int rc = dpx_stage_cells(fb.graph, W, H, fb.diagram, fb.ec, nullptr)) return rc;
+    if (subdivide) {
+        if (int rc = dpx_stage_subdivide(fb.img, W, H, stride, ch, flags, fb.graph, fb.diagram, fb.diagram_aux, fb.ec,
+                                         fb.ec_aux, nullptr))
+            return rc;
+    }
+    if (int rc = dpx_stage_triangulate(fb.diagram, fb.ec, W, H, nullptr)) return rc;
+    (void)N;
+    return DPX_OK;
+}
+
+int alloc_frame(FrameBuffers& fb, int W, int H, int stride, bool subdivide)
+{
+    size_t N = (size_t)W * H;
+    DPX_HIP(hipMalloc(&fb.img, (size_t)stride * H));
+    DPX_HIP(hipMalloc(&fb.graph, N));
+    DPX_HIP(hipMalloc(&fb.graph_aux, N));
+    DPX_HIP(hipMalloc(&fb.diagram, N * dpx::kCellSize * sizeof(dpx_point)));
+    DPX_HIP(hipMalloc(&fb.ec, N * sizeof(int32_t)));
+    if (subdivide) {
+        DPX_HIP(hipMalloc(&fb.diagram_aux, N * dpx::kCellSize * sizeof(dpx_point)));
+        DPX_HIP(hipMalloc(&fb.ec_aux, N * sizeof(int32_t)));
+    }
+    return DPX_OK;
+}
+
+}  // namespace
+
+extern "C" int dpx_frame(const uint8_t* img, int W, int H, int stride, int ch, const dpx_params* params,
+                         uint8_t* out_rgba, int out_stride, uint8_t* graph_out, int32_t* edge_count_out,
+                         dpx_point* diagram_out)
+{
+    using namespace dpx;
+    if (int rc = check_img(img, W, H, stride, ch)) return rc;
+    DPX_CHECK(params, DPX_EINVAL, "null params");
+    DPX_CHECK(params->reserved == 0, DPX_EINVAL, "reserved must be 0");
+    if (int rc = require_device()) return rc;
+    int scale = params->scale;
+    bool sub = params->subdivide != 0;
+    if (out_rgba) {
+        DPX_CHECK(scale >= 1 && scale <= 32, DPX_EINVAL, "scale %d outside 1..32", scale);
+        DPX_CHECK(out_stride >= 4 * W * scale, DPX_EINVAL, "bad out_stride %d", out_stride);
+    }
+    size_t N = (size_t)W * H;
+    FrameBuffers fb;
+    if (int rc = alloc_frame(fb, W, H, stride, sub)) return rc;
+    DPX_HIP(hipMemcpy(fb.img, img, (size_t)stride * H, hipMemcpyHostToDevice));
+    if (int rc = run_stages(fb, W, H, stride, ch, params->flags, sub)) return rc;
+    if (out_rgba) {
+        DPX_HIP(hipMalloc(&fb.out, (size_t)out_stride * H * scale));
+        if (int rc = dpx_stage_rasterize(fb.img, W, H, stride, ch, params->flags, fb.diagram, fb.ec, scale, fb.out,
+                                         out_stride, nullptr))
+            return rc;
+    }
+    DPX_HIP(hipDeviceSynchronize());
+    if (out_rgba) DPX_HIP(hipMemcpy(out_rgba, fb.out, (size_t)out_stride * H * scale, hipMemcpyDeviceToHost));
+    if (graph_out) DPX_HIP(hipMemcpy(graph_out, fb.graph, N, hipMemcpyDeviceToHost));
+    if (edge_count_out) DPX_HIP(hipMemcpy(edge_count_out, fb.ec, N * sizeof(int32_t), hipMemcpyDeviceToHost));
+    if (diagram_out)
+        DPX_HIP(hipMemcpy(diagram_out, fb.diagram, N * kCellSize * sizeof(dpx_point), hipMemcpyDeviceToHost));
+    return DPX_OK;
+}
+
+extern "C" dpx_point* launch_kernel(dpx_point* pos, dpx_uchar4* colorPos, float time, char* img_data, int img_width,
+                                    int img_height, int img_widthstep, int* edge_count_h, char* graph_h,
+                                    bool subdivide)
+{
+    using namespace dpx;
+    (void)time;
+    int W = img_width, H = img_height;
+    if (check_img(img_data, W, H, img_widthstep, 3) || require_device()) return nullptr;
+    size_t N = (size_t)W * H;
+    FrameBuffers fb;
+    if (alloc_frame(fb, W, H, img_widthstep, subdivide)) return nullptr;
+    if (hipMemcpy(fb.img, img_data, (size_t)img_widthstep * H, hipMemcpyHostToDevice) != hipSuccess) {
+        fail(DPX_EHIP, "image upload failed");
+        return nullptr;
+    }
+    if (run_stages(fb, W, H, img_widthstep, 3, 0, subdivide)) return nullptr;
+    if (pos || colorPos) {
+        if (dpx_stage_export_vbo(fb.img, W, H, img_widthstep, 3, 0, fb.diagram, fb.ec, pos, colorPos, nullptr))
+            return nullptr;
+    }
+    if (hipDeviceSynchronize() != hipSuccess) {
+        fail(DPX_EHIP, "device error in the stage kernels");
+        return nullptr;
+    }
+    dpx_point* diagram_h = (dpx_point*)malloc(N * kCellSize * sizeof(dpx_point));
+    if (!diagram_h) {
+        fail(DPX_ENOMEM, "host allocation of the diagram failed");
+        return nullptr;
+    }
+    bool ok = true;
+    if (edge_count_h)
+        ok = ok && hipMemcpy(edge_count_h, fb.ec, N * sizeof(int32_t), hipMemcpyDeviceToHost) == hipSuccess;
+    ok = ok && hipMemcpy(diagram_h, fb.diagram, N * kCellSize * sizeof(dpx_point), hipMemcpyDeviceToHost) == hipSuccess;
+    if (graph_h) ok = ok && hipMemcpy(graph_h, fb.graph, N, hipMemcpyDeviceToHost) == hipSuccess;
+    if (!ok) {
+        free(diagram_h);
+        fail(DPX_EHIP, "result download failed");
+        return nullptr;
+    }
+    return diagram_h;
+}

@@ -1,0 +1,238 @@
+"""ctypes binding of libdepixel.so, the MI355X depixelization path.
+
+The library is built in-tree (``make -C pixel-art-remaster-gpu_amd``); this
+module refuses to import without it, so nothing silently falls back to a CPU
+path.  Device buffers are plain pointers: the helpers below take torch HIP
+tensors (``tensor.data_ptr()``) and torch streams (``stream.cuda_stream``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from ctypes import POINTER, c_bool, c_char_p, c_double, c_float, c_int, c_uint32, c_uint64, c_void_p
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "build", "libdepixel.so")
+HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "depixel.h")
+
+CELL_SIZE = 45
+RGB, TOPDOWN, ALPHA = 1, 2, 4
+
+OK, EINVAL, ENOMEM, EHIP, ENODEV, EBUDGET, ETIMEOUT, ECANCELLED, ECLOSED, EINJECTED = (
+    0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
+
+
+class Point(ctypes.Structure):
+    _fields_ = [("x", c_float), ("y", c_float)]
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("scale", c_int), ("subdivide", c_int), ("flags", c_uint32), ("reserved", c_int)]
+
+
+class Item(ctypes.Structure):
+    _fields_ = [("in_offset", c_uint64), ("out_offset", c_uint64), ("width", c_int), ("height", c_int),
+                ("stride", c_int), ("channels", c_int), ("scale", c_int), ("subdivide", c_int),
+                ("flags", c_uint32), ("out_stride", c_int)]
+
+
+class BatchTimes(ctypes.Structure):
+    _fields_ = [("graph_ms", c_double), ("heuristic_ms", c_double), ("raster_ms", c_double), ("batches", c_uint64)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("submitted", c_uint64), ("completed", c_uint64), ("failed", c_uint64), ("rejected", c_uint64),
+                ("cancelled", c_uint64), ("batches", c_uint64), ("frames_per_s", c_double), ("p50_ms", c_double),
+                ("p99_ms", c_double), ("mean_batch", c_double), ("hbm_in_use", c_uint64), ("hbm_peak", c_uint64),
+                ("hbm_budget", c_uint64)]
+
+
+class DepixelError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"depixel error {code}: {msg}")
+        self.code = code
+
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(f"{LIB_PATH} is not built; run `make -C pixel-art-remaster-gpu_amd` "
+                      "(or __graft_entry__.build()) first")
+
+lib = ctypes.CDLL(LIB_PATH)
+
+_u8p = POINTER(ctypes.c_uint8)
+_SIGS = {
+    "dpx_last_error": (c_char_p, []),
+    "dpx_version": (c_char_p, []),
+    "launch_kernel": (c_void_p, [c_void_p, c_void_p, c_float, c_void_p, c_int, c_int, c_int, c_void_p, c_void_p,
+                                 c_bool]),
+    "dpx_frame": (c_int, [c_void_p, c_int, c_int, c_int, c_int, POINTER(Params), c_void_p, c_int, c_void_p,
+                          c_void_p, c_void_p]),
+    "dpx_stage_similarity_graph": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_uint32, c_void_p, c_void_p]),
+    "dpx_stage_trivial_cross": (c_int, [c_void_p, c_int, c_int, c_void_p]),
+    "dpx_stage_heuristic_cross": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "dpx_stage_cells": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    "dpx_stage_subdivide": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_uint32, c_void_p, c_void_p, c_void_p,
+                                    c_void_p, c_void_p, c_void_p]),
+    "dpx_stage_triangulate": (c_int, [c_void_p, c_void_p, c_int, c_int, c_void_p]),
+    "dpx_stage_export_vbo": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_uint32, c_void_p, c_void_p, c_void_p,
+                                     c_void_p, c_void_p]),
+    "dpx_stage_rasterize": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_uint32, c_void_p, c_void_p, c_int,
+                                    c_void_p, c_int, c_void_p]),
+    "dpx_stage_components": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "dpx_context_create": (c_int, [c_int, POINTER(c_void_p)]),
+    "dpx_context_destroy": (None, [c_void_p]),
+    "dpx_batch_run": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(Item), c_int, c_void_p]),
+    "dpx_batch_graph": (c_int, [c_void_p, POINTER(c_void_p), POINTER(POINTER(c_uint64))]),
+    "dpx_batch_timing": (c_int, [c_void_p, c_int]),
+    "dpx_batch_timing_read": (c_int, [c_void_p, POINTER(BatchTimes)]),
+    "dpx_server_create": (c_int, [c_int, c_uint64, c_int, POINTER(c_void_p)]),
+    "dpx_server_destroy": (None, [c_void_p]),
+    "dpx_server_open_stream": (c_int, [c_void_p, c_int, c_int, c_int, POINTER(Params), POINTER(c_int)]),
+    "dpx_server_close_stream": (c_int, [c_void_p, c_int]),
+    "dpx_server_submit": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(c_uint64)]),
+    "dpx_server_wait": (c_int, [c_void_p, c_uint64, c_int]),
+    "dpx_server_cancel": (c_int, [c_void_p, c_uint64]),
+    "dpx_server_stats_get": (c_int, [c_void_p, POINTER(ServerStats)]),
+    "dpx_server_inject_failure": (c_int, [c_void_p, c_uint64]),
+    "dpx_synth_frame": (c_int, [c_int, c_uint64, c_int, c_int, c_int, c_void_p, c_int]),
+    "dpx_synth_sprite_size": (c_int, [c_uint64, c_int, POINTER(c_int), POINTER(c_int)]),
+}
+for _name, (_res, _args) in _SIGS.items():
+    if hasattr(lib, _name):
+        _f = getattr(lib, _name)
+        _f.restype = _res
+        _f.argtypes = _args
+
+libc = ctypes.CDLL(None)
+libc.free.argtypes = [c_void_p]
+
+
+def last_error() -> str:
+    return (lib.dpx_last_error() or b"").decode(errors="replace")
+
+
+def check(rc: int) -> int:
+    if rc != OK:
+        raise DepixelError(rc, last_error())
+    return rc
+
+
+def _ptr(a) -> int:
+    """Address of a numpy array or a torch tensor (host or device)."""
+    if a is None:
+        return 0
+    if isinstance(a, np.ndarray):
+        assert a.flags["C_CONTIGUOUS"]
+        return a.ctypes.data
+    return a.data_ptr()
+
+
+def _stream(s) -> int:
+    if s is None:
+        return 0
+    return getattr(s, "cuda_stream", s)
+
+
+# --------------------------------------------------------------------------
+# synthetic inputs
+# --------------------------------------------------------------------------
+def synth_frame(kind: int, seed: int, width: int, height: int, channels: int = 3,
+                stride: int | None = None) -> np.ndarray:
+    """Deterministic synthetic frame as a (height, stride) uint8 array."""
+    if stride is None:
+        stride = (width * channels + 3) // 4 * 4
+    buf = np.zeros((height, stride), dtype=np.uint8)
+    check(lib.dpx_synth_frame(kind, seed, width, height, channels, _ptr(buf), stride))
+    return buf
+
+
+def sprite_size(seed: int, index: int) -> tuple[int, int]:
+    w, h = c_int(), c_int()
+    check(lib.dpx_synth_sprite_size(seed, index, ctypes.byref(w), ctypes.byref(h)))
+    return w.value, h.value
+
+
+# --------------------------------------------------------------------------
+# plain per-frame path
+# --------------------------------------------------------------------------
+def frame(img: np.ndarray, width: int, height: int, channels: int = 3, scale: int = 4, subdivide: bool = False,
+          flags: int = 0, want_diagram: bool = False):
+    """Runs dpx_frame on a host frame; returns (rgba (sH, sW, 4), graph (H, W), edge_count (H, W)[, diagram])."""
+    stride = img.shape[1]
+    out = np.zeros((height * scale, width * scale * 4), dtype=np.uint8)
+    graph = np.zeros((height, width), dtype=np.uint8)
+    ec = np.zeros((height, width), dtype=np.int32)
+    diag = np.zeros((height * width * CELL_SIZE, 2), dtype=np.float32) if want_diagram else None
+    p = Params(scale, int(subdivide), flags, 0)
+    check(lib.dpx_frame(_ptr(img), width, height, stride, channels, ctypes.byref(p), _ptr(out), out.shape[1],
+                        _ptr(graph), _ptr(ec), _ptr(diag)))
+    rgba = out.reshape(height * scale, width * scale, 4)
+    if want_diagram:
+        return rgba, graph, ec, diag
+    return rgba, graph, ec
+
+
+def launch_kernel(img: np.ndarray, width: int, height: int, subdivide: bool = False, pos=None, color=None):
+    """The reference-compatible drop-in; returns (diagram (W*H*45, 2), edge_count, graph)."""
+    stride = img.shape[1]
+    ec = np.zeros(width * height, dtype=np.int32)
+    graph = np.zeros(width * height, dtype=np.uint8)
+    ptr = lib.launch_kernel(_ptr(pos), _ptr(color), 0.0, _ptr(img), width, height, stride, _ptr(ec), _ptr(graph),
+                            bool(subdivide))
+    if not ptr:
+        raise DepixelError(-1, last_error())
+    n = width * height * CELL_SIZE * 2
+    diag = np.ctypeslib.as_array(ctypes.cast(ptr, POINTER(c_float)), shape=(n,)).copy().reshape(-1, 2)
+    libc.free(ptr)
+    return diag, ec, graph
+
+
+# --------------------------------------------------------------------------
+# batched path
+# --------------------------------------------------------------------------
+class Context:
+    def __init__(self, device: int = 0):
+        h = c_void_p()
+        check(lib.dpx_context_create(device, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib.dpx_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, d_in, d_out, items: list[Item], stream=None):
+        arr = (Item * len(items))(*items)
+        check(lib.dpx_batch_run(self.h, _ptr(d_in), _ptr(d_out), arr, len(items), _stream(stream)))
+
+    def timing(self, enable: bool):
+        check(lib.dpx_batch_timing(self.h, int(enable)))
+
+    def timing_read(self) -> BatchTimes:
+        t = BatchTimes()
+        check(lib.dpx_batch_timing_read(self.h, ctypes.byref(t)))
+        return t
+
+    def graph(self, n_items: int):
+        """(device pointer of the final graphs, per-item byte offsets) of the last batch."""
+        p = c_void_p()
+        offs = POINTER(c_uint64)()
+        check(lib.dpx_batch_graph(self.h, ctypes.byref(p), ctypes.byref(offs)))
+        return p.value, [offs[k] for k in range(n_items)]
+
+
+def header_symbols() -> list[str]:
+    """Function names declared in include/depixel.h."""
+    import re
+    src = open(HEADER_PATH).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"^[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", src, flags=re.M)
+    return sorted(set(n for n in names if n not in ("defined",)))

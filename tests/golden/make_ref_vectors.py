@@ -1,0 +1,100 @@
+"""Writes golden vectors produced by the reference's own per-pixel device
+functions (oracle/_ref/libdepixel_ref.so, built from the reference sources by
+`make -C oracle ref`).  Runs on a GPU:
+
+    python tests/golden/make_ref_vectors.py OUT_DIR
+
+writes OUT_DIR/ref_cell_table.npz and OUT_DIR/ref_frames.npz, which are
+committed under tests/golden/.  Inputs are small seeded synthetic frames
+(bottom-up BGR with padded rows, the reference's layout) and the two resolved
+graphs the reference ships.  Only slots the reference defines are stored
+(hull points 0..count, the first (count-2)*3 triangle slots).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "pixel-art-remaster-gpu_amd"))
+
+import oracle as O  # noqa: E402
+import depixel as D  # noqa: E402
+
+CS = 45
+FRAMES = [  # (kind, seed, W, H, row padding bytes)
+    (0, 11, 40, 32, 4),
+    (0, 12, 37, 29, 8),
+    (2, 13, 40, 32, 4),
+    (2, 14, 33, 27, 12),
+    (4, 15, 40, 32, 4),
+    (4, 16, 36, 30, 4),
+]
+
+
+def ragged(d, counts, width_fn):
+    """Concatenate the defined slots of each cell."""
+    parts, lens = [], []
+    for k, n in enumerate(counts):
+        m = max(0, width_fn(int(n)))
+        parts.append(d[k, :m])
+        lens.append(m)
+    return np.concatenate(parts).astype(np.float32), np.array(lens, dtype=np.int32)
+
+
+def main(out_dir):
+    os.makedirs(out_dir, exist_ok=True)
+    pts, cnt = O.ref_cell_table()
+    np.savez_compressed(os.path.join(out_dir, "ref_cell_table.npz"), points=pts[:, :17].copy(), counts=cnt)
+    print("cell table: max count", int(cnt.max()), flush=True)
+
+    rec = {}
+    graphs = json.load(open(os.path.join(ROOT, "tests", "golden", "reference_graphs.json")))
+    for name in ("c_pattern", "alex"):
+        g = np.array(graphs[name]["graph"], dtype=np.uint8).reshape(graphs[name]["height"], graphs[name]["width"])
+        H, W = g.shape
+        d, ec = O.ref_cells(g)
+        cp, cl = ragged(d, ec, lambda n: n + 1)
+        t = O.ref_triangulate(d.reshape(-1, 2), ec, W, H).reshape(-1, CS, 2)
+        tp, tl = ragged(t, ec, lambda n: (n - 2) * 3)
+        rec[f"{name}_graph"] = g
+        rec[f"{name}_cells_pts"], rec[f"{name}_cells_len"], rec[f"{name}_cells_count"] = cp, cl, ec
+        rec[f"{name}_tri_pts"], rec[f"{name}_tri_len"] = tp, tl
+        print(name, "cells ok", flush=True)
+
+    for fi, (kind, seed, W, H, pad) in enumerate(FRAMES):
+        stride = W * 3 + pad
+        img = D.synth_frame(kind, seed, W, H, 3, stride)
+        g0 = O.ref_graph(img, W, H)
+        g1 = O.ref_trivial(g0)
+        g2 = O.ref_heuristic(g1)
+        blocks = np.array([(i, j) for j in range(H - 1) for i in range(W - 1)
+                           if (g1[j, i] >> 2) & 1 and (g1[j + 1, i + 1] >> 5) & 1 and (g1[j, i + 1] >> 0) & 1
+                           and (g1[j + 1, i] >> 7) & 1], dtype=np.int32).reshape(-1, 2)
+        dec = O.ref_decisions(g1, blocks) if len(blocks) else np.zeros(0, np.int32)
+        d, ec = O.ref_cells(g2)
+        cp, cl = ragged(d, ec, lambda n: n + 1)
+        ds, es = O.ref_subdivide(img, W, H, g2, d.reshape(-1, 2), ec)
+        ds = ds.reshape(-1, CS, 2)
+        sp, sl = ragged(ds, es, lambda n: n)
+        t = O.ref_triangulate(ds.reshape(-1, 2), es, W, H).reshape(-1, CS, 2)
+        tp, tl = ragged(t, es, lambda n: (n - 2) * 3)
+        p = f"f{fi}_"
+        rec[p + "meta"] = np.array([kind, seed, W, H, stride], dtype=np.int32)
+        rec[p + "img"] = img
+        rec[p + "graph_raw"], rec[p + "graph_trivial"], rec[p + "graph_final"] = g0, g1, g2
+        rec[p + "blocks"], rec[p + "decisions"] = blocks, dec
+        rec[p + "cells_pts"], rec[p + "cells_len"], rec[p + "cells_count"] = cp, cl, ec
+        rec[p + "sub_pts"], rec[p + "sub_len"], rec[p + "sub_count"] = sp, sl, es
+        rec[p + "tri_pts"], rec[p + "tri_len"] = tp, tl
+        print(f"frame {fi}: {W}x{H} kind {kind}: {len(blocks)} crossing blocks, "
+              f"{int((g2 != g1).sum())} heuristic changes", flush=True)
+    rec["n_frames"] = np.array(len(FRAMES))
+    np.savez_compressed(os.path.join(out_dir, "ref_frames.npz"), **rec)
+    print("wrote", out_dir, flush=True)
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else os.path.join(ROOT, "gpurun_out", "golden"))

@@ -1,0 +1,238 @@
+"""GPU parity: every HIP path against the oracle on the same seeded inputs.
+
+Bar: bit-exact graph, edge counts, cell/triangle vertices and output pixels.
+"""
+import numpy as np
+import pytest
+
+pytestmark = [pytest.mark.gpu, pytest.mark.timeout(300)]
+
+CS = 45
+
+# (kind, seed, W, H, channels, flags, scale, subdivide, row padding)
+FRAME_CASES = [
+    (0, 1, 256, 224, 3, 0, 4, False, 0),     # the 256x224 -> 4x workload
+    (0, 2, 256, 224, 3, 0, 4, True, 0),
+    (0, 3, 61, 47, 3, 0, 3, True, 8),        # ragged size, padded rows
+    (2, 4, 96, 64, 3, 1, 4, False, 4),       # RGB order
+    (2, 5, 80, 72, 3, 2, 2, True, 0),        # top-down rows
+    (4, 6, 120, 90, 3, 0, 8, True, 4),
+    (1, 7, 24, 24, 4, 4, 6, False, 0),       # sprite with alpha
+    (1, 8, 48, 64, 4, 6, 6, True, 0),        # sprite, alpha + top-down
+    (0, 9, 1, 1, 3, 0, 4, False, 0),         # single pixel
+    (0, 10, 7, 1, 3, 0, 5, True, 3),         # single row
+    (2, 11, 1, 9, 3, 2, 3, True, 0),         # single column
+]
+
+
+def make_frame(D, kind, seed, W, H, ch, pad):
+    stride = W * ch + pad
+    return D.synth_frame(kind, seed, W, H, ch, stride)
+
+
+def assert_equal(a, b, what):
+    a, b = np.asarray(a), np.asarray(b)
+    assert a.shape == b.shape, f"{what}: shape {a.shape} vs {b.shape}"
+    bad = np.argwhere(a != b)
+    assert len(bad) == 0, f"{what}: {len(bad)} mismatches, first at {bad[:3].tolist()}"
+
+
+def defined_slots(diag, ec, width_fn):
+    """{cell: slots} of the slots a stage defines."""
+    d = diag.reshape(-1, CS, 2)
+    return [d[k, :max(0, width_fn(int(n)))] for k, n in enumerate(np.asarray(ec).reshape(-1))]
+
+
+@pytest.mark.parametrize("case", FRAME_CASES, ids=lambda c: f"k{c[0]}s{c[1]}_{c[2]}x{c[3]}c{c[4]}f{c[5]}x{c[6]}sub{int(c[7])}")
+def test_plain_frame_matches_oracle(gpu, D, O, case):
+    kind, seed, W, H, ch, flags, scale, sub, pad = case
+    print(f"case {case}", flush=True)
+    img = make_frame(D, kind, seed, W, H, ch, pad)
+    ref = O.depixelize(img, W, H, scale=scale, subdivide=sub, channels=ch, flags=flags, want_diagram=True)
+    rgba, graph, ec, diag = D.frame(img, W, H, ch, scale=scale, subdivide=sub, flags=flags, want_diagram=True)
+    assert_equal(graph, ref["graph"], "graph")
+    assert_equal(ec, ref["edge_count"], "edge_count")
+    got = defined_slots(diag, ec, lambda n: (n - 2) * 3)
+    exp = defined_slots(ref["diagram"], ref["edge_count"], lambda n: (n - 2) * 3)
+    for k, (a, b) in enumerate(zip(got, exp)):
+        assert np.array_equal(a, b), f"triangles of cell {k} differ"
+    assert_equal(rgba, ref["rgba"], "rgba")
+
+
+def test_stage_entry_points_match_oracle(gpu, D, O):
+    torch = gpu
+    W, H, S = 83, 61, 4
+    img = D.synth_frame(2, 21, W, H, 3, W * 3 + 4)
+    stride = img.shape[1]
+    dev = "cuda:0"
+    s = torch.cuda.current_stream()
+    d_img = torch.from_numpy(img.reshape(-1)).to(dev)
+    d_g = torch.zeros(W * H, dtype=torch.uint8, device=dev)
+    d_ga = torch.zeros(W * H, dtype=torch.uint8, device=dev)
+    D.check(D.lib.dpx_stage_similarity_graph(d_img.data_ptr(), W, H, stride, 3, 0, d_g.data_ptr(), s.cuda_stream))
+    torch.cuda.synchronize()
+    g0 = O.similarity_graph(img, W, H)
+    assert_equal(d_g.cpu().numpy().reshape(H, W), g0, "similarity graph")
+    D.check(D.lib.dpx_stage_trivial_cross(d_g.data_ptr(), W, H, s.cuda_stream))
+    torch.cuda.synchronize()
+    g1 = O.trivial_cross(g0)
+    assert_equal(d_g.cpu().numpy().reshape(H, W), g1, "trivial cross")
+    D.check(D.lib.dpx_stage_heuristic_cross(d_g.data_ptr(), d_ga.data_ptr(), W, H, s.cuda_stream))
+    torch.cuda.synchronize()
+    g2 = O.heuristic_cross(g1)
+    assert_equal(d_g.cpu().numpy().reshape(H, W), g2, "heuristic cross")
+    assert_equal(d_ga.cpu().numpy().reshape(H, W), g1, "heuristic snapshot")
+
+    d_diag = torch.zeros(W * H * CS * 2, dtype=torch.float32, device=dev)
+    d_daux = torch.zeros_like(d_diag)
+    d_ec = torch.zeros(W * H, dtype=torch.int32, device=dev)
+    d_eaux = torch.zeros_like(d_ec)
+    D.check(D.lib.dpx_stage_cells(d_g.data_ptr(), W, H, d_diag.data_ptr(), d_ec.data_ptr(), s.cuda_stream))
+    torch.cuda.synchronize()
+    od, oec = O.cells(g2)
+    assert_equal(d_ec.cpu().numpy(), oec, "cell counts")
+    for k, (a, b) in enumerate(zip(defined_slots(d_diag.cpu().numpy(), oec, lambda n: n + 1),
+                                   defined_slots(od.reshape(-1, 2), oec, lambda n: n + 1))):
+        assert np.array_equal(a, b), f"hull of cell {k}"
+
+    D.check(D.lib.dpx_stage_subdivide(d_img.data_ptr(), W, H, stride, 3, 0, d_g.data_ptr(), d_diag.data_ptr(),
+                                      d_daux.data_ptr(), d_ec.data_ptr(), d_eaux.data_ptr(), s.cuda_stream))
+    torch.cuda.synchronize()
+    sd, sec, _, _ = O.subdivide(img, W, H, g2, od, oec)
+    assert_equal(d_ec.cpu().numpy(), sec, "subdivided counts")
+    for k, (a, b) in enumerate(zip(defined_slots(d_diag.cpu().numpy(), sec, lambda n: n),
+                                   defined_slots(sd.reshape(-1, 2), sec, lambda n: n))):
+        assert np.array_equal(a, b), f"subdivided polygon of cell {k}"
+
+    D.check(D.lib.dpx_stage_triangulate(d_diag.data_ptr(), d_ec.data_ptr(), W, H, s.cuda_stream))
+    torch.cuda.synchronize()
+    td, _ = O.triangulate(sd, sec, W, H)
+    assert_equal(d_diag.cpu().numpy().reshape(-1), td.reshape(-1), "triangulated diagram (all 45 slots)")
+
+    d_pos = torch.zeros(W * H * CS * 2, dtype=torch.float32, device=dev)
+    d_col = torch.zeros(W * H * CS * 4, dtype=torch.uint8, device=dev)
+    D.check(D.lib.dpx_stage_export_vbo(d_img.data_ptr(), W, H, stride, 3, 0, d_diag.data_ptr(), d_ec.data_ptr(),
+                                       d_pos.data_ptr(), d_col.data_ptr(), s.cuda_stream))
+    torch.cuda.synchronize()
+    pos, col = O.export_vbo(img, W, H, td, sec)
+    assert_equal(d_pos.cpu().numpy().reshape(-1, 2), pos, "VBO positions")
+    assert_equal(d_col.cpu().numpy().reshape(-1, 4), col, "VBO colours")
+
+    d_out = torch.zeros(H * S * W * S * 4, dtype=torch.uint8, device=dev)
+    D.check(D.lib.dpx_stage_rasterize(d_img.data_ptr(), W, H, stride, 3, 0, d_diag.data_ptr(), d_ec.data_ptr(), S,
+                                      d_out.data_ptr(), W * S * 4, s.cuda_stream))
+    torch.cuda.synchronize()
+    ras = O.rasterize(img, W, H, td, sec, S)
+    assert_equal(d_out.cpu().numpy().reshape(H * S, W * S, 4), ras, "raster")
+
+    d_lab = torch.zeros(W * H, dtype=torch.int32, device=dev)
+    D.check(D.lib.dpx_stage_components(d_g.data_ptr(), W, H, d_lab.data_ptr(), s.cuda_stream))
+    torch.cuda.synchronize()
+    assert_equal(d_lab.cpu().numpy().reshape(H, W), O.components(g2), "component labels")
+
+
+def test_launch_kernel_dropin(gpu, D, O):
+    torch = gpu
+    W, H = 64, 40
+    img = D.synth_frame(0, 31, W, H, 3, W * 3)
+    for sub in (False, True):
+        pos = torch.zeros(W * H * CS * 2, dtype=torch.float32, device="cuda:0")
+        col = torch.zeros(W * H * CS * 4, dtype=torch.uint8, device="cuda:0")
+        diag, ec, graph = D.launch_kernel(img, W, H, sub, pos, col)
+        ref = O.depixelize(img, W, H, scale=1, subdivide=sub, want_diagram=True)
+        assert_equal(graph.reshape(H, W), ref["graph"], "graph_h")
+        assert_equal(ec.reshape(H, W), ref["edge_count"], "edge_count_h")
+        assert_equal(diag.reshape(-1), ref["diagram"].reshape(-1), "returned diagram")
+        opos, ocol = O.export_vbo(img, W, H, ref["diagram"], ref["edge_count"].reshape(-1))
+        assert_equal(pos.cpu().numpy().reshape(-1, 2), opos, "pos VBO")
+        assert_equal(col.cpu().numpy().reshape(-1, 4), ocol, "colorPos VBO")
+
+
+_hip = None
+
+
+def hip_d2h(host, dev_ptr, n):
+    """hipMemcpy device -> host of a raw device pointer (the context's graph arena)."""
+    global _hip
+    import ctypes
+    if _hip is None:
+        _hip = ctypes.CDLL("libamdhip64.so")
+        _hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    return _hip.hipMemcpy(host.ctypes.data, dev_ptr, n, 2)  # hipMemcpyDeviceToHost
+
+
+def run_batch(torch, D, frames, items_spec):
+    """frames: list of (img, W, H, ch); items_spec: list of (scale, sub, flags)."""
+    in_off, out_off, items = 0, 0, []
+    blobs = []
+    for (img, W, H, ch), (scale, sub, flags) in zip(frames, items_spec):
+        out_stride = W * scale * 4
+        items.append(D.Item(in_off, out_off, W, H, img.shape[1], ch, scale, int(sub), flags, out_stride))
+        blobs.append(img.reshape(-1))
+        in_off += img.size
+        in_off = (in_off + 255) // 256 * 256
+        blobs.append(np.zeros(in_off - sum(b.size for b in blobs), dtype=np.uint8))
+        out_off += out_stride * H * scale
+        out_off = (out_off + 255) // 256 * 256
+    d_in = torch.from_numpy(np.concatenate(blobs)).to("cuda:0")
+    d_out = torch.zeros(out_off, dtype=torch.uint8, device="cuda:0")
+    ctx = D.Context(0)
+    ctx.run(d_in, d_out, items)
+    torch.cuda.synchronize()
+    out = d_out.cpu().numpy()
+    gptr, goffs = ctx.graph(len(items))
+    graphs = []
+    for it, go in zip(items, goffs):
+        n = it.width * it.height
+        host = np.zeros(n, dtype=np.uint8)
+        assert hip_d2h(host, gptr + go, n) == 0
+        graphs.append(host.reshape(it.height, it.width))
+    res = []
+    for it in items:
+        o = out[it.out_offset:it.out_offset + it.out_stride * it.height * it.scale]
+        res.append(o.reshape(it.height * it.scale, it.out_stride)[:, :it.width * it.scale * 4]
+                   .reshape(it.height * it.scale, it.width * it.scale, 4))
+    ctx.close()
+    return res, graphs
+
+
+def test_batched_mixed_items_match_oracle(gpu, D, O):
+    torch = gpu
+    frames, spec, refs = [], [], []
+    cases = [
+        (0, 41, 256, 224, 3, 0, 4, False, 0),
+        (0, 42, 256, 224, 3, 0, 4, True, 0),
+        (2, 43, 320, 224, 3, 0, 4, False, 0),
+        (2, 44, 240, 160, 3, 2, 4, True, 4),
+        (4, 45, 160, 120, 3, 0, 8, True, 0),
+        (0, 46, 63, 37, 3, 1, 5, False, 8),
+        (0, 47, 2, 2, 3, 0, 15, True, 0),
+        (0, 48, 50, 30, 3, 0, 1, False, 0),
+        (0, 49, 50, 30, 3, 0, 7, True, 0),
+    ]
+    for s in range(10):
+        w, h = D.sprite_size(5, s)
+        cases.append((1, 100 + s, w, h, 4, 4 | (2 if s % 3 == 0 else 0), 6, s % 2 == 1, 0))
+    for kind, seed, W, H, ch, flags, scale, sub, pad in cases:
+        img = make_frame(D, kind, seed, W, H, ch, pad)
+        frames.append((img, W, H, ch))
+        spec.append((scale, sub, flags))
+        refs.append(O.depixelize(img, W, H, scale=scale, subdivide=sub, channels=ch, flags=flags))
+    outs, graphs = run_batch(torch, D, frames, spec)
+    for k, (o, g, r, c) in enumerate(zip(outs, graphs, refs, cases)):
+        print(f"item {k}: {c}", flush=True)
+        assert_equal(g, r["graph"], f"item {k} graph")
+        assert_equal(o, r["rgba"], f"item {k} rgba")
+
+
+def test_batch_result_independent_of_batch_mix(gpu, D):
+    """The same frame alone and inside a mixed batch gives identical bytes."""
+    torch = gpu
+    img = D.synth_frame(0, 77, 256, 224, 3)
+    alone, _ = run_batch(torch, D, [(img, 256, 224, 3)], [(4, False, 0)])
+    others = [(D.synth_frame(2, 78 + k, 96 + 8 * k, 64 + 4 * k, 3), 96 + 8 * k, 64 + 4 * k, 3) for k in range(5)]
+    frames = others[:2] + [(img, 256, 224, 3)] + others[2:]
+    spec = [(4, k % 2 == 0, 0) for k in range(len(frames))]
+    spec[2] = (4, False, 0)
+    mixed, _ = run_batch(torch, D, frames, spec)
+    assert_equal(mixed[2], alone[0], "frame alone vs in a batch")

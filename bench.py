@@ -37,6 +37,8 @@ def main():
     ap.add_argument("--plain-frames", type=int, default=20)
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a plain 1 GiB device copy (reference for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames are independent, "
@@ -112,11 +114,21 @@ def main():
     parity_ok = bool(np.array_equal(got, rgba.reshape(-1)))
 
     # plain per-frame path: one dpx_frame call per frame (host in/out, per-call allocation)
-    D.frame(host[0], W, H, 3, scale=S)
-    t0 = time.perf_counter()
-    for f in range(args.plain_frames):
-        D.frame(host[f % max_b], W, H, 3, scale=S)
-    plain_fps = args.plain_frames / (time.perf_counter() - t0)
+    plain_fps = None
+    if args.plain_frames > 0:
+        D.frame(host[0], W, H, 3, scale=S)
+        t0 = time.perf_counter()
+        for f in range(args.plain_frames):
+            D.frame(host[f % max_b], W, H, 3, scale=S)
+        plain_fps = args.plain_frames / (time.perf_counter() - t0)
+
+    if args.calibrate_copy:
+        src = torch.ones(1 << 30, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
 
     traffic = None
     tp = os.path.join(ROOT, "profiles", "raster_traffic.json")
@@ -158,8 +170,8 @@ def main():
                    "frame": "256x224", "scale": S},
         "p50_ms": round(float(np.percentile(lat, 50)), 4),
         "p99_ms": round(float(np.percentile(lat, 99)), 4),
-        "plain_frames_per_s": round(plain_fps, 2),
-        "speedup_vs_plain": round(fps / plain_fps, 1),
+        "plain_frames_per_s": round(plain_fps, 2) if plain_fps else None,
+        "speedup_vs_plain": round(fps / plain_fps, 1) if plain_fps else None,
         "kernel_ms": {"graph": round(graph_ms, 5), "heuristic": round(heur_ms, 5), "raster": round(raster_ms, 5)},
         "parity_spot_check": parity_ok,
         "sweep": sweep,

@@ -1,2 +1,469 @@
-// server.cpp -- placeholder translation unit, replaced by the serving loop.
+// server.cpp -- serving loop with continuous batching under an HBM budget.
+//
+// The server reserves its HBM budget as one device arena when it is created.
+// Requests (one host frame each, from any number of open streams) queue in
+// arrival order; a request is admitted when its input and output buffers can
+// be carved out of the arena (first fit, oldest first).  Two lanes, each with
+// its own HIP stream, batched-path context and pinned staging buffers, take
+// admitted work whenever they are free: a lane forms a batch of up to
+// max_batch requests, uploads the frames, runs the fused kernel chain,
+// downloads the results and completes the tickets, releasing their arena
+// ranges.  While one lane's batch is on the device the other admits the
+// requests that arrived meanwhile, so requests overlap on the GPU as memory
+// frees up.  Each frame's result depends only on that frame (the kernels keep
+// no state across frames), so it is the same alone and in any mix.
+#include <string.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
 #include "common.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+size_t align256(size_t v) { return (v + 255) / 256 * 256; }
+
+// First-fit allocator of byte ranges inside the device arena.
+class Arena {
+public:
+    void reset(size_t size)
+    {
+        free_.clear();
+        free_[0] = size;
+        used_ = 0;
+    }
+    bool alloc(size_t n, size_t* off)
+    {
+        n = align256(n);
+        for (auto it = free_.begin(); it != free_.end(); ++it) {
+            if (it->second < n) continue;
+            size_t o = it->first, rest = it->second - n;
+            free_.erase(it);
+            if (rest) free_[o + n] = rest;
+            *off = o;
+            used_ += n;
+            return true;
+        }
+        return false;
+    }
+    void release(size_t off, size_t n)
+    {
+        n = align256(n);
+        used_ -= n;
+        auto it = free_.emplace(off, n).first;
+        auto next = std::next(it);
+        if (next != free_.end() && it->first + it->second == next->first) {
+            it->second += next->second;
+            free_.erase(next);
+        }
+        if (it != free_.begin()) {
+            auto prev = std::prev(it);
+            if (prev->first + prev->second == it->first) {
+                prev->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+    size_t used() const { return used_; }
+
+private:
+    std::map<size_t, size_t> free_;
+    size_t used_ = 0;
+};
+
+struct StreamInfo {
+    int W, H, ch;
+    dpx_params params;
+    bool open;
+};
+
+struct Request {
+    uint64_t ticket = 0;
+    uint64_t index = 0;  // submission order
+    int stream_id = 0;
+    int W = 0, H = 0, ch = 0;
+    dpx_params params{};
+    const uint8_t* img = nullptr;
+    int stride = 0;
+    uint8_t* out = nullptr;
+    int out_stride = 0;
+    size_t in_bytes = 0, out_bytes = 0;
+    size_t in_off = 0, out_off = 0;  // arena ranges while admitted
+    Clock::time_point t_submit;
+    int status = 1;  // 1 pending, 2 admitted, else the final dpx_status
+    bool inject = false;
+};
+
+struct Lane {
+    dpx_context* ctx = nullptr;
+    hipStream_t stream = nullptr;
+    uint8_t *h_in = nullptr, *h_out = nullptr;
+    size_t hin_cap = 0, hout_cap = 0;
+    std::thread worker;
+};
+
+}  // namespace
+
+struct dpx_server {
+    int device = 0;
+    uint64_t budget = 0;
+    int max_batch = 64;
+    uint8_t* d_arena = nullptr;
+    Arena arena;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    bool closing = false;
+    std::map<int, StreamInfo> streams;
+    int next_stream = 1;
+    std::deque<Request*> pending;
+    std::map<uint64_t, std::unique_ptr<Request>> requests;
+    uint64_t next_ticket = 1, next_index = 0;
+    std::vector<uint64_t> inject_at;
+    uint64_t peak = 0;
+    uint64_t submitted = 0, completed = 0, failed = 0, rejected = 0, cancelled = 0, batches = 0, batched_frames = 0;
+    std::vector<double> latencies_ms;
+    Clock::time_point t_create;
+    Lane lanes[2];
+};
+
+namespace {
+
+using namespace dpx;
+
+int grow_pinned(uint8_t** p, size_t* cap, size_t need)
+{
+    if (need <= *cap) return DPX_OK;
+    size_t n = std::max(need, *cap * 3 / 2);
+    if (*p) (void)hipHostFree(*p);
+    *p = nullptr;
+    *cap = 0;
+    DPX_HIP(hipHostMalloc(p, n, hipHostMallocDefault));
+    *cap = n;
+    return DPX_OK;
+}
+
+// Runs one admitted batch on a lane; returns a status for the whole batch.
+int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
+{
+    std::vector<dpx_item> items;
+    std::vector<Request*> live;
+    size_t hin = 0, hout = 0;
+    for (Request* r : batch) {
+        if (r->inject) continue;
+        live.push_back(r);
+        hin += align256(r->in_bytes);
+        hout += align256(r->out_bytes);
+    }
+    if (live.empty()) return DPX_OK;
+    if (int rc = grow_pinned(&lane.h_in, &lane.hin_cap, hin)) return rc;
+    if (int rc = grow_pinned(&lane.h_out, &lane.hout_cap, hout)) return rc;
+    size_t ho = 0;
+    for (Request* r : live) {
+        memcpy(lane.h_in + ho, r->img, r->in_bytes);
+        DPX_HIP(hipMemcpyAsync(srv->d_arena + r->in_off, lane.h_in + ho, r->in_bytes, hipMemcpyHostToDevice,
+                               lane.stream));
+        ho += align256(r->in_bytes);
+        dpx_item it;
+        memset(&it, 0, sizeof(it));
+        it.in_offset = r->in_off;
+        it.out_offset = r->out_off;
+        it.width = r->W;
+        it.height = r->H;
+        it.stride = r->stride;
+        it.channels = r->ch;
+        it.scale = r->params.scale;
+        it.subdivide = r->params.subdivide;
+        it.flags = r->params.flags;
+        it.out_stride = r->W * r->params.scale * 4;
+        items.push_back(it);
+    }
+    if (int rc = dpx_batch_run(lane.ctx, srv->d_arena, srv->d_arena, items.data(), (int)items.size(), lane.stream))
+        return rc;
+    ho = 0;
+    std::vector<size_t> host_off;
+    for (Request* r : live) {
+        DPX_HIP(hipMemcpyAsync(lane.h_out + ho, srv->d_arena + r->out_off, r->out_bytes, hipMemcpyDeviceToHost,
+                               lane.stream));
+        host_off.push_back(ho);
+        ho += align256(r->out_bytes);
+    }
+    DPX_HIP(hipStreamSynchronize(lane.stream));
+    for (size_t k = 0; k < live.size(); k++) {
+        Request* r = live[k];
+        int row = r->W * r->params.scale * 4, rows = r->H * r->params.scale;
+        const uint8_t* src = lane.h_out + host_off[k];
+        for (int y = 0; y < rows; y++) memcpy(r->out + (size_t)y * r->out_stride, src + (size_t)y * row, (size_t)row);
+    }
+    return DPX_OK;
+}
+
+// Admits the oldest pending requests that fit (caller holds the lock).
+void admit(dpx_server* srv, std::vector<Request*>& batch)
+{
+    while (!srv->pending.empty() && (int)batch.size() < srv->max_batch) {
+        Request* r = srv->pending.front();
+        size_t a, b;
+        if (!srv->arena.alloc(r->in_bytes, &a)) break;
+        if (!srv->arena.alloc(r->out_bytes, &b)) {
+            srv->arena.release(a, r->in_bytes);
+            break;
+        }
+        r->in_off = a;
+        r->out_off = b;
+        r->status = 2;
+        srv->pending.pop_front();
+        batch.push_back(r);
+    }
+    srv->peak = std::max<uint64_t>(srv->peak, srv->arena.used());
+}
+
+void lane_loop(dpx_server* srv, int li)
+{
+    Lane& lane = srv->lanes[li];
+    (void)hipSetDevice(srv->device);
+    for (;;) {
+        std::vector<Request*> batch;
+        {
+            std::unique_lock<std::mutex> lock(srv->mu);
+            for (;;) {
+                if (srv->closing && srv->pending.empty()) return;
+                if (!srv->pending.empty()) {
+                    admit(srv, batch);
+                    if (!batch.empty()) break;
+                    // the arena is held by the other lane's batch: wait for it
+                    srv->cv_done.wait(lock);
+                } else {
+                    srv->cv_work.wait(lock);
+                }
+            }
+        }
+        int rc = run_batch(srv, lane, batch);
+        {
+            std::lock_guard<std::mutex> lock(srv->mu);
+            auto now = Clock::now();
+            srv->batches++;
+            srv->batched_frames += batch.size();
+            for (Request* r : batch) {
+                srv->arena.release(r->in_off, r->in_bytes);
+                srv->arena.release(r->out_off, r->out_bytes);
+                r->status = r->inject ? DPX_EINJECTED : rc;
+                if (r->status == DPX_OK) {
+                    srv->completed++;
+                    srv->latencies_ms.push_back(
+                        std::chrono::duration<double, std::milli>(now - r->t_submit).count());
+                } else {
+                    srv->failed++;
+                }
+            }
+        }
+        srv->cv_done.notify_all();
+        srv->cv_work.notify_all();
+    }
+}
+
+double percentile(std::vector<double> v, double q)
+{
+    if (v.empty()) return 0.0;
+    std::sort(v.begin(), v.end());
+    double pos = q * (double)(v.size() - 1);
+    size_t lo = (size_t)pos;
+    size_t hi = std::min(lo + 1, v.size() - 1);
+    return v[lo] + (v[hi] - v[lo]) * (pos - (double)lo);
+}
+
+}  // namespace
+
+extern "C" int dpx_server_create(int device, uint64_t hbm_budget_bytes, int max_batch, dpx_server** out)
+{
+    DPX_CHECK(out, DPX_EINVAL, "null output pointer");
+    *out = nullptr;
+    DPX_CHECK(hbm_budget_bytes >= 4096 && max_batch > 0 && max_batch <= 65536, DPX_EINVAL,
+              "bad budget or batch size");
+    if (int rc = require_device()) return rc;
+    DPX_HIP(hipSetDevice(device));
+    auto* srv = new dpx_server();
+    srv->device = device;
+    srv->budget = align256(hbm_budget_bytes);
+    srv->max_batch = max_batch;
+    srv->t_create = Clock::now();
+    if (hipMalloc(&srv->d_arena, srv->budget) != hipSuccess) {
+        dpx_server_destroy(srv);
+        return fail(DPX_ENOMEM, "cannot reserve the HBM budget (%llu bytes)", (unsigned long long)srv->budget);
+    }
+    srv->arena.reset(srv->budget);
+    for (Lane& lane : srv->lanes) {
+        int rc = dpx_context_create(device, &lane.ctx);
+        if (rc == DPX_OK && hipStreamCreateWithFlags(&lane.stream, hipStreamNonBlocking) != hipSuccess)
+            rc = fail(DPX_EHIP, "stream creation failed");
+        if (rc != DPX_OK) {
+            dpx_server_destroy(srv);
+            return rc;
+        }
+    }
+    for (int k = 0; k < 2; k++) srv->lanes[k].worker = std::thread(lane_loop, srv, k);
+    *out = srv;
+    return DPX_OK;
+}
+
+extern "C" void dpx_server_destroy(dpx_server* srv)
+{
+    if (!srv) return;
+    {
+        std::lock_guard<std::mutex> lock(srv->mu);
+        srv->closing = true;
+    }
+    srv->cv_work.notify_all();
+    srv->cv_done.notify_all();
+    for (Lane& lane : srv->lanes) {
+        if (lane.worker.joinable()) lane.worker.join();
+        if (lane.stream) (void)hipStreamSynchronize(lane.stream);
+        if (lane.h_in) (void)hipHostFree(lane.h_in);
+        if (lane.h_out) (void)hipHostFree(lane.h_out);
+        if (lane.stream) (void)hipStreamDestroy(lane.stream);
+        dpx_context_destroy(lane.ctx);
+    }
+    (void)hipFree(srv->d_arena);
+    delete srv;
+}
+
+extern "C" int dpx_server_open_stream(dpx_server* srv, int width, int height, int channels, const dpx_params* params,
+                                      int* stream_id)
+{
+    DPX_CHECK(srv && params && stream_id, DPX_EINVAL, "bad arguments");
+    DPX_CHECK(width > 0 && height > 0 && width <= 32768 && height <= 32768, DPX_EINVAL, "bad size %dx%d", width,
+              height);
+    DPX_CHECK(channels == 3 || channels == 4, DPX_EINVAL, "channels must be 3 or 4");
+    DPX_CHECK(params->scale >= 1 && params->scale <= 15, DPX_EINVAL, "scale %d outside 1..15", params->scale);
+    DPX_CHECK(params->reserved == 0, DPX_EINVAL, "reserved must be 0");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    DPX_CHECK(!srv->closing, DPX_ECLOSED, "server closing");
+    int id = srv->next_stream++;
+    srv->streams[id] = StreamInfo{ width, height, channels, *params, true };
+    *stream_id = id;
+    return DPX_OK;
+}
+
+extern "C" int dpx_server_close_stream(dpx_server* srv, int stream_id)
+{
+    DPX_CHECK(srv, DPX_EINVAL, "null server");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    auto it = srv->streams.find(stream_id);
+    DPX_CHECK(it != srv->streams.end() && it->second.open, DPX_EINVAL, "unknown stream %d", stream_id);
+    it->second.open = false;  // requests already submitted still complete
+    return DPX_OK;
+}
+
+extern "C" int dpx_server_submit(dpx_server* srv, int stream_id, const uint8_t* img, int stride, uint8_t* out,
+                                 int out_stride, uint64_t* ticket)
+{
+    DPX_CHECK(srv && ticket, DPX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    DPX_CHECK(!srv->closing, DPX_ECLOSED, "server closing");
+    auto it = srv->streams.find(stream_id);
+    DPX_CHECK(it != srv->streams.end() && it->second.open, DPX_ECLOSED, "stream %d is not open", stream_id);
+    const StreamInfo& si = it->second;
+    DPX_CHECK(img && out, DPX_EINVAL, "null frame or output buffer");
+    DPX_CHECK(stride >= si.W * si.ch, DPX_EINVAL, "stride %d < %d", stride, si.W * si.ch);
+    DPX_CHECK(out_stride >= si.W * si.params.scale * 4, DPX_EINVAL, "out_stride %d too small", out_stride);
+    auto r = std::make_unique<Request>();
+    r->W = si.W;
+    r->H = si.H;
+    r->ch = si.ch;
+    r->params = si.params;
+    r->in_bytes = (size_t)stride * si.H;
+    r->out_bytes = (size_t)si.W * si.params.scale * 4 * si.H * si.params.scale;
+    if (align256(r->in_bytes) + align256(r->out_bytes) > srv->budget) {
+        srv->rejected++;
+        return fail(DPX_EBUDGET, "frame needs %zu bytes of HBM, budget is %llu", r->in_bytes + r->out_bytes,
+                    (unsigned long long)srv->budget);
+    }
+    r->ticket = srv->next_ticket++;
+    r->index = srv->next_index++;
+    r->stream_id = stream_id;
+    r->img = img;
+    r->stride = stride;
+    r->out = out;
+    r->out_stride = out_stride;
+    r->t_submit = Clock::now();
+    r->inject = std::find(srv->inject_at.begin(), srv->inject_at.end(), r->index) != srv->inject_at.end();
+    *ticket = r->ticket;
+    srv->pending.push_back(r.get());
+    srv->requests[r->ticket] = std::move(r);
+    srv->submitted++;
+    srv->cv_work.notify_one();
+    return DPX_OK;
+}
+
+extern "C" int dpx_server_wait(dpx_server* srv, uint64_t ticket, int timeout_ms)
+{
+    DPX_CHECK(srv, DPX_EINVAL, "null server");
+    std::unique_lock<std::mutex> lock(srv->mu);
+    auto it = srv->requests.find(ticket);
+    DPX_CHECK(it != srv->requests.end(), DPX_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
+    Request* r = it->second.get();
+    auto done = [&] { return r->status <= 0; };
+    if (timeout_ms < 0) srv->cv_done.wait(lock, done);
+    else if (!srv->cv_done.wait_for(lock, std::chrono::milliseconds(timeout_ms), done))
+        return fail(DPX_ETIMEOUT, "ticket %llu still pending", (unsigned long long)ticket);
+    int st = r->status;
+    srv->requests.erase(it);
+    if (st != DPX_OK) return fail(st, "request %llu failed (%d)", (unsigned long long)ticket, st);
+    return DPX_OK;
+}
+
+extern "C" int dpx_server_cancel(dpx_server* srv, uint64_t ticket)
+{
+    DPX_CHECK(srv, DPX_EINVAL, "null server");
+    {
+        std::lock_guard<std::mutex> lock(srv->mu);
+        auto it = srv->requests.find(ticket);
+        DPX_CHECK(it != srv->requests.end(), DPX_EINVAL, "unknown ticket %llu", (unsigned long long)ticket);
+        Request* r = it->second.get();
+        DPX_CHECK(r->status == 1, DPX_EINVAL, "ticket %llu is already admitted or done", (unsigned long long)ticket);
+        srv->pending.erase(std::find(srv->pending.begin(), srv->pending.end(), r));
+        r->status = DPX_ECANCELLED;
+        srv->cancelled++;
+    }
+    srv->cv_done.notify_all();
+    srv->cv_work.notify_all();
+    return DPX_OK;
+}
+
+extern "C" int dpx_server_stats_get(dpx_server* srv, dpx_server_stats* out)
+{
+    DPX_CHECK(srv && out, DPX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    memset(out, 0, sizeof(*out));
+    out->submitted = srv->submitted;
+    out->completed = srv->completed;
+    out->failed = srv->failed;
+    out->rejected = srv->rejected;
+    out->cancelled = srv->cancelled;
+    out->batches = srv->batches;
+    double secs = std::chrono::duration<double>(Clock::now() - srv->t_create).count();
+    out->frames_per_s = secs > 0 ? (double)srv->completed / secs : 0.0;
+    out->p50_ms = percentile(srv->latencies_ms, 0.50);
+    out->p99_ms = percentile(srv->latencies_ms, 0.99);
+    out->mean_batch = srv->batches ? (double)srv->batched_frames / (double)srv->batches : 0.0;
+    out->hbm_in_use = srv->arena.used();
+    out->hbm_peak = srv->peak;
+    out->hbm_budget = srv->budget;
+    return DPX_OK;
+}
+
+extern "C" int dpx_server_inject_failure(dpx_server* srv, uint64_t request_index)
+{
+    DPX_CHECK(srv, DPX_EINVAL, "null server");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    srv->inject_at.push_back(request_index);
+    return DPX_OK;
+}

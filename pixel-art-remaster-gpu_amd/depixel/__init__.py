@@ -229,6 +229,64 @@ class Context:
         return p.value, [offs[k] for k in range(n_items)]
 
 
+class Server:
+    """Serving loop: continuous batching of host frames under an HBM budget."""
+
+    def __init__(self, budget_bytes: int, max_batch: int = 64, device: int = 0):
+        h = c_void_p()
+        check(lib.dpx_server_create(device, budget_bytes, max_batch, ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def close(self):
+        if self.h:
+            lib.dpx_server_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def open_stream(self, width: int, height: int, channels: int = 3, scale: int = 4, subdivide: bool = False,
+                    flags: int = 0) -> int:
+        sid = c_int()
+        p = Params(scale, int(subdivide), flags, 0)
+        check(lib.dpx_server_open_stream(self.h, width, height, channels, ctypes.byref(p), ctypes.byref(sid)))
+        return sid.value
+
+    def close_stream(self, sid: int):
+        check(lib.dpx_server_close_stream(self.h, sid))
+
+    def submit(self, sid: int, img: np.ndarray, out: np.ndarray) -> int:
+        """Queues a host frame; `out` (rows of >= 4*scale*W bytes) is filled on completion."""
+        t = c_uint64()
+        rc = lib.dpx_server_submit(self.h, sid, _ptr(img), img.shape[1], _ptr(out), out.shape[1], ctypes.byref(t))
+        check(rc)
+        self._keep[t.value] = (img, out)
+        return t.value
+
+    def wait(self, ticket: int, timeout_ms: int = -1) -> int:
+        """Returns the request's status code (0 = done); raises nothing."""
+        rc = lib.dpx_server_wait(self.h, ticket, timeout_ms)
+        if rc != ETIMEOUT:
+            self._keep.pop(ticket, None)
+        return rc
+
+    def cancel(self, ticket: int) -> int:
+        return lib.dpx_server_cancel(self.h, ticket)
+
+    def inject_failure(self, request_index: int):
+        check(lib.dpx_server_inject_failure(self.h, request_index))
+
+    def stats(self) -> ServerStats:
+        s = ServerStats()
+        check(lib.dpx_server_stats_get(self.h, ctypes.byref(s)))
+        return s
+
+
 def header_symbols() -> list[str]:
     """Function names declared in include/depixel.h."""
     import re

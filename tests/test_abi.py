@@ -1,0 +1,69 @@
+"""The C-ABI library loads on a machine without a GPU, exports every function
+include/depixel.h declares, and refuses work cleanly when no GPU is visible."""
+import ctypes
+import os
+import re
+
+import pytest
+
+from conftest import ROOT
+
+
+def declared_functions():
+    src = open(os.path.join(ROOT, "include", "depixel.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    names = re.findall(r"\b([a-z_][a-z0-9_]*)\s*\(", src)
+    keywords = {"if", "while", "for", "return", "sizeof", "defined"}
+    return sorted(set(n for n in names if n not in keywords))
+
+
+def test_header_declares_the_drop_in_and_the_stages():
+    names = declared_functions()
+    for n in ("launch_kernel", "dpx_frame", "dpx_stage_similarity_graph", "dpx_stage_trivial_cross",
+              "dpx_stage_heuristic_cross", "dpx_stage_cells", "dpx_stage_subdivide", "dpx_stage_triangulate",
+              "dpx_stage_export_vbo", "dpx_stage_rasterize", "dpx_stage_components", "dpx_batch_run",
+              "dpx_server_submit"):
+        assert n in names
+
+
+def test_library_exports_every_declared_function(D):
+    lib = ctypes.CDLL(D.LIB_PATH)
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+    assert D.lib.dpx_version().startswith(b"depixel")
+
+
+def test_synthetic_generator_is_deterministic(D):
+    import numpy as np
+    a = D.synth_frame(0, 5, 256, 224, 3)
+    b = D.synth_frame(0, 5, 256, 224, 3)
+    c = D.synth_frame(0, 6, 256, 224, 3)
+    assert np.array_equal(a, b) and not np.array_equal(a, c)
+    sizes = {D.sprite_size(9, k) for k in range(200)}
+    assert sizes == {(8, 8), (16, 16), (16, 32), (24, 24), (32, 32), (48, 64)}
+    s = D.synth_frame(1, 3, 24, 24, 4)
+    alpha = s.reshape(24, 24, 4)[..., 3]
+    assert 0.15 < (alpha == 0).mean() < 0.45
+
+
+def test_argument_checks_need_no_gpu(D):
+    import numpy as np
+    img = np.zeros((4, 12), dtype=np.uint8)
+    p = D.Params(4, 0, 0, 0)
+    # bad channel count and bad stride are rejected before any device call
+    assert D.lib.dpx_frame(img.ctypes.data, 4, 4, 12, 2, ctypes.byref(p), None, 0, None, None, None) == D.EINVAL
+    assert D.lib.dpx_frame(img.ctypes.data, 4, 4, 8, 3, ctypes.byref(p), None, 0, None, None, None) == D.EINVAL
+    assert "stride" in D.last_error()
+
+
+@pytest.mark.skipif(os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"),
+                    reason="a GPU may be visible")
+def test_no_device_is_reported(D):
+    import numpy as np
+    img = np.zeros((4, 12), dtype=np.uint8)
+    p = D.Params(4, 0, 0, 0)
+    rc = D.lib.dpx_frame(img.ctypes.data, 4, 4, 12, 3, ctypes.byref(p), None, 0, None, None, None)
+    assert rc == D.ENODEV
+    h = ctypes.c_void_p()
+    assert D.lib.dpx_context_create(0, ctypes.byref(h)) == D.ENODEV
+    assert D.lib.dpx_server_create(0, 1 << 20, 8, ctypes.byref(h)) == D.ENODEV

@@ -104,7 +104,6 @@ def main():
     ms_per_step = wall * 1e3 / args.steps
     raster_ms = times.raster_ms / max(1, times.batches)
     graph_ms = times.graph_ms / max(1, times.batches)
-    heur_ms = times.heuristic_ms / max(1, times.batches)
     achieved = ALGO_BYTES * best_b / (raster_ms * 1e-3) / 1e9
 
     # parity spot check of the timed output against the per-frame path
@@ -172,7 +171,7 @@ def main():
         "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "plain_frames_per_s": round(plain_fps, 2) if plain_fps else None,
         "speedup_vs_plain": round(fps / plain_fps, 1) if plain_fps else None,
-        "kernel_ms": {"graph": round(graph_ms, 5), "heuristic": round(heur_ms, 5), "raster": round(raster_ms, 5)},
+        "kernel_ms": {"graph": round(graph_ms, 5), "raster": round(raster_ms, 5)},
         "parity_spot_check": parity_ok,
         "sweep": sweep,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -167,8 +167,9 @@ int dpx_batch_graph(dpx_context* ctx, const uint8_t** d_graph, const uint64_t** 
  * stream around each of its kernels.  Off by default; reading waits for the
  * recorded batches, returns the accumulated times and resets them. */
 typedef struct {
-    double graph_ms, heuristic_ms, raster_ms;  /* summed over the batches        */
-    uint64_t batches;                            /* batches recorded since reset   */
+    double graph_ms;    /* similarity graph + trivial crossings, summed over the batches */
+    double raster_ms;   /* heuristic crossings + cells + raster, summed over the batches */
+    uint64_t batches;   /* batches recorded since the last read                           */
 } dpx_batch_times;
 int dpx_batch_timing(dpx_context* ctx, int enable);
 int dpx_batch_timing_read(dpx_context* ctx, dpx_batch_times* out);

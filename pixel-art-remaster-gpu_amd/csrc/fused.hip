@@ -1,19 +1,22 @@
 // fused.hip -- the batched path: many frames or sprites of uneven size
-// through three fused kernels.
+// through two fused kernels.
 //
-//   kf_graph   LDS tile 64x16 (+2 halo): YUV once per pixel, 8 similarity bits,
-//              trivial crossing removal -> snapshot graph (1 B/px)
-//   kf_heur    heuristic crossing resolution on the snapshot -> final graph
-//   kf_raster  tile of 32x8 cells: cell keys (and, with subdivision, hulls,
-//              smoothed polygons and their coverage) in LDS, then every output
-//              row segment of a cell resolves its S pixels against the 3x3
-//              candidate cells in draw order and streams RGBA8 to HBM.
+//   kf_graph   LDS tile 64x16 (+2 halo): YUV once per pixel, similarity in
+//              four directions (the other four by symmetry), trivial
+//              crossing removal -> snapshot graph (1 B/px)
+//   kf_raster  tile of 32x8 cells: heuristic crossing resolution on the
+//              snapshot (tile + halo 3), cell keys (and, with subdivision,
+//              hulls, smoothed polygons and their coverage) in LDS; cells
+//              whose S*S pixels are all their own colour stream them to HBM
+//              at once, the rest are listed and resolved row by row against
+//              the 3x3 candidate cells in draw order.
 //
 // Coverage of an unsmoothed cell depends only on its 12-bit key and the
 // scale, so it is precomputed once per scale into a table of sample-row
 // bitmasks (kt_mask): bit b of row r of key k says whether sample
 // (kmin+b, kmin+r) of the cell's local sample grid is covered, kmin =
-// -(S/2)-1, MW = 2S+2 rows and columns (local [-0.5, 1.5] of the cell).
+// -(S/2)-1, MW = 2S+2 rows and columns (local [-0.5, 1.5] of the cell); and
+// into a 16-bit summary per key (kt_info, coverage_info).
 #include <stdlib.h>
 #include <string.h>
 
@@ -99,8 +102,47 @@ __global__ void kt_mask(const int8_t* __restrict__ hull_table, int S, uint32_t* 
     mask[gid] = coverage_row(x, y, tri, nt, S, r);
 }
 
+// Summary of a cell's coverage rows at scale S: bit 0 = it covers every
+// sample of its own square; bit 1 + (dy+1)*3 + (dx+1) = it covers a sample of
+// the neighbouring square at (dx, dy).
+__host__ __device__ inline uint32_t coverage_info(const uint32_t* rows, int S)
+{
+    const int kmin = mask_kmin(S), mw = mask_w(S);
+    const int c0 = -kmin;  // bit of the own square's first column
+    const uint32_t own = ((1u << S) - 1u) << c0;
+    uint32_t info = 0;
+    bool full = true;
+    for (int r = 0; r < mw; r++) {
+        int ky = kmin + r;
+        int dy = ky < 0 ? -1 : (ky >= S ? 1 : 0);
+        uint32_t row = rows[r];
+        if (dy == 0 && (row & own) != own) full = false;
+        if (row & ((1u << c0) - 1u)) info |= 1u << (1 + (dy + 1) * 3 + 0);
+        if ((row & own) && dy != 0) info |= 1u << (1 + (dy + 1) * 3 + 1);
+        if (row >> (c0 + S)) info |= 1u << (1 + (dy + 1) * 3 + 2);
+    }
+    return info | (full ? 1u : 0u);
+}
+
+__global__ void kt_info(const uint32_t* __restrict__ mask, int S, uint16_t* __restrict__ info)
+{
+    int key = blockIdx.x * blockDim.x + threadIdx.x;
+    if (key >= kKeys) return;
+    info[key] = (uint16_t)coverage_info(mask + (size_t)key * mask_w(S), S);
+}
+
+// A cell whose own square it covers fully, and into which none of the four
+// cells drawn after it (up-left, up, up-right, right) reaches, shows its own
+// colour on all of its S*S output pixels.
+__device__ inline bool simple_cell(uint32_t own, uint32_t upl, uint32_t up, uint32_t upr, uint32_t right)
+{
+    return (own & 1u) && !((upl >> 3) & 1u) && !((up >> 2) & 1u) && !((upr >> 1) & 1u) && !((right >> 4) & 1u);
+}
+
 // ---------------------------------------------------------------------------
-// work distribution: block b -> (item, tile) through a prefix of tile counts
+// work distribution: block b -> (item, tile).  When every item of a launch
+// has the same number of tiles (`uniform`), it is a division; otherwise a
+// binary search over the prefix of tile counts.
 // ---------------------------------------------------------------------------
 __device__ inline int find_item(const uint32_t* __restrict__ prefix, int n, uint32_t b)
 {
@@ -112,6 +154,18 @@ __device__ inline int find_item(const uint32_t* __restrict__ prefix, int n, uint
     return lo;
 }
 
+__device__ inline void block_item(const uint32_t* __restrict__ prefix, int n, uint32_t uniform, uint32_t b,
+                                  int* item, int* tile)
+{
+    if (uniform) {
+        *item = (int)(b / uniform);
+        *tile = (int)(b % uniform);
+    } else {
+        *item = find_item(prefix, n, b);
+        *tile = (int)(b - prefix[*item]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // kf_graph: similarity graph + trivial crossing removal
 // ---------------------------------------------------------------------------
@@ -120,17 +174,18 @@ constexpr int GLX = GTX + 4, GLY = GTY + 4;  // YUV tile, halo 2
 constexpr int GBX = GTX + 2, GBY = GTY + 2;  // bit tile, halo 1
 
 __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix,
-                                                int n_items, const uint8_t* __restrict__ in, uint8_t* __restrict__ gaux)
+                                                int n_items, uint32_t uniform, const uint8_t* __restrict__ in,
+                                                uint8_t* __restrict__ gaux)
 {
     __shared__ uint32_t s_yuv[GLX * GLY];
-    __shared__ uint8_t s_cls[GLX * GLY];  // 0 opaque, 1 transparent, 2 outside the frame
+    __shared__ uint8_t s_cls[GLX * GLY];   // 0 opaque, 1 transparent, 2 outside the frame
+    __shared__ uint8_t s_half[GLX * GLY];  // bits 1, 2, 4, 7 (up, up-right, right, down-right)
     __shared__ uint8_t s_bits[GBX * GBY];
     const int t = threadIdx.x;
-    const uint32_t b = blockIdx.x;
-    const int ii = find_item(prefix, n_items, b);
+    int ii, tile;
+    block_item(prefix, n_items, uniform, blockIdx.x, &ii, &tile);
     const DevItem it = items[ii];
     const int tiles_x = (it.W + GTX - 1) / GTX;
-    const int tile = (int)(b - prefix[ii]);
     const int x0 = (tile % tiles_x) * GTX, y0 = (tile / tiles_x) * GTY;
     ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags };
 
@@ -145,20 +200,33 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         }
     }
     __syncthreads();
-    for (int k = t; k < GBX * GBY; k += 256) {
-        int lx = k % GBX + 1, ly = k / GBX + 1;
-        int c = ly * GLX + lx;
+    // Similarity is symmetric: test the four directions up, up-right, right
+    // and down-right once per pixel, and take the other four from the
+    // neighbours' bits below.
+    for (int k = t; k < GLX * GLY; k += 256) {
+        int lx = k % GLX, ly = k / GLX;
         uint32_t bits = 0;
-        if (s_cls[c] != 2) {
-            Yuv a{ s_yuv[c], s_cls[c] };
-#pragma unroll
-            for (int e = 0; e < 8; e++) {
-                int q = c + edy(e) * GLX + edx(e);
-                if (s_cls[q] == 2) continue;
-                Yuv bb{ s_yuv[q], s_cls[q] };
-                if (similar(a, bb)) bits |= 1u << e;
-            }
+        if (s_cls[k] != 2) {
+            Yuv a{ s_yuv[k], s_cls[k] };
+            auto test = [&](int q, int e) {
+                if (s_cls[q] != 2 && similar(a, Yuv{ s_yuv[q], s_cls[q] })) bits |= 1u << e;
+            };
+            bool up = ly + 1 < GLY, rt = lx + 1 < GLX;
+            if (up) test(k + GLX, 1);
+            if (up && rt) test(k + GLX + 1, 2);
+            if (rt) test(k + 1, 4);
+            if (rt && ly >= 1) test(k - GLX + 1, 7);
         }
+        s_half[k] = (uint8_t)bits;
+    }
+    __syncthreads();
+    for (int k = t; k < GBX * GBY; k += 256) {
+        int c = (k / GBX + 1) * GLX + k % GBX + 1;
+        uint32_t bits = s_half[c];
+        bits |= (((uint32_t)s_half[c - 1] >> 4) & 1u) << 3;        // left: its right bit
+        bits |= (((uint32_t)s_half[c - GLX] >> 1) & 1u) << 6;      // down: its up bit
+        bits |= (((uint32_t)s_half[c - GLX - 1] >> 2) & 1u) << 5;  // down-left: its up-right bit
+        bits |= (((uint32_t)s_half[c + GLX - 1] >> 7) & 1u) << 0;  // up-left: its down-right bit
         s_bits[k] = (uint8_t)bits;
     }
     __syncthreads();
@@ -176,82 +244,85 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
 }
 
 // ---------------------------------------------------------------------------
-// kf_heur: heuristic crossing resolution on the snapshot
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void kf_heur(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix,
-                                               int n_items, const uint8_t* __restrict__ gaux, uint8_t* __restrict__ gfin)
-{
-    __shared__ uint8_t s_g[GBX * GBY];
-    const int t = threadIdx.x;
-    const uint32_t b = blockIdx.x;
-    const int ii = find_item(prefix, n_items, b);
-    const DevItem it = items[ii];
-    const int tiles_x = (it.W + GTX - 1) / GTX;
-    const int tile = (int)(b - prefix[ii]);
-    const int x0 = (tile % tiles_x) * GTX, y0 = (tile / tiles_x) * GTY;
-    const uint8_t* aux = gaux + it.g_off;
-    for (int k = t; k < GBX * GBY; k += 256) {
-        int x = x0 - 1 + k % GBX, y = y0 - 1 + k / GBX;
-        s_g[k] = (x >= 0 && y >= 0 && x < it.W && y < it.H) ? aux[(size_t)y * it.W + x] : 0;
-    }
-    __syncthreads();
-    uint8_t* g = gfin + it.g_off;
-    for (int k = t; k < GTX * GTY; k += 256) {
-        int lx = k % GTX, ly = k / GTX;
-        int x = x0 + lx, y = y0 + ly;
-        if (x >= it.W || y >= it.H) continue;
-        int c = (ly + 1) * GBX + lx + 1;
-        uint32_t own = s_g[c];
-        uint32_t out = own;
-        if (own & 0xA5u) {  // any diagonal bit (0, 2, 5, 7)
-            uint32_t nb[8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) nb[e] = s_g[c + edy(e) * GBX + edx(e)];
-            out = own & ~heuristic_clear_mask(aux, it.W, x, y, own, nb);
-        }
-        g[(size_t)y * it.W + x] = (uint8_t)out;
-    }
-}
-
-// ---------------------------------------------------------------------------
-// kf_raster: cells (+ subdivision) + gather raster
+// kf_raster: heuristic crossing resolution, cells (+ subdivision) and the
+// gather raster of a 32x8-cell tile
 // ---------------------------------------------------------------------------
 constexpr int RTX = 32, RTY = 8;
-constexpr int RGX = RTX + 6, RGY = RTY + 6;  // graph bytes, halo 3
+constexpr int RAX = RTX + 8, RAY = RTY + 8;  // snapshot graph bytes, halo 4
+constexpr int RGX = RTX + 6, RGY = RTY + 6;  // final graph bytes, halo 3
 constexpr int RHX = RTX + 4, RHY = RTY + 4;  // hulls, halo 2 (subdivision)
 constexpr int RCX = RTX + 2, RCY = RTY + 2;  // candidate cells, halo 1
 constexpr uint16_t kNoCell = 0xFFFF;
 
 template <int S, bool SUB>
 __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const int32_t* __restrict__ group,
-                                                 const uint32_t* __restrict__ prefix, int n_group,
-                                                 const uint8_t* __restrict__ in, const uint8_t* __restrict__ gfin,
-                                                 const int8_t* __restrict__ hull_table,
-                                                 const uint32_t* __restrict__ mask_table, uint8_t* __restrict__ out)
+                                                 const uint32_t* __restrict__ prefix, int n_group, uint32_t uniform,
+                                                 const uint8_t* __restrict__ in, const uint8_t* __restrict__ gaux,
+                                                 uint8_t* __restrict__ gfin, const int8_t* __restrict__ hull_table,
+                                                 const uint32_t* __restrict__ mask_table,
+                                                 const uint16_t* __restrict__ info_table, uint8_t* __restrict__ out)
 {
     constexpr int MW = mask_w(S), KMIN = mask_kmin(S);
+    __shared__ uint8_t s_aux[RAX * RAY];
     __shared__ uint8_t s_g[RGX * RGY];
     __shared__ uint16_t s_key[RCX * RCY];
+    __shared__ uint16_t s_info[RCX * RCY];
     __shared__ uint32_t s_col[RCX * RCY];
-    __shared__ uint32_t s_mask[RCX * RCY * MW];
+    __shared__ uint32_t s_mask[RCX * RCY * MW];  // coverage rows of the candidates that need them
     __shared__ int8_t s_hull[SUB ? RHX * RHY * 17 : 1];
+    __shared__ uint16_t s_list[RTX * RTY];  // own cells that need the candidate search
+    __shared__ uint8_t s_need[RCX * RCY];   // candidates of listed cells
+    __shared__ int s_nlist;
 
     const int t = threadIdx.x;
-    const uint32_t b = blockIdx.x;
-    const int gi = find_item(prefix, n_group, b);
+    int gi, tile;
+    block_item(prefix, n_group, uniform, blockIdx.x, &gi, &tile);
     const DevItem it = items[group[gi]];
     const int tiles_x = (it.W + RTX - 1) / RTX;
-    const int tile = (int)(b - prefix[gi]);
     const int x0 = (tile % tiles_x) * RTX, y0 = (tile / tiles_x) * RTY;
     ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags };
-    const uint8_t* g = gfin + it.g_off;
+    const uint8_t* aux = gaux + it.g_off;
 
-    for (int k = t; k < RGX * RGY; k += 256) {
-        int x = x0 - 3 + k % RGX, y = y0 - 3 + k / RGX;
-        s_g[k] = (x >= 0 && y >= 0 && x < it.W && y < it.H) ? g[(size_t)y * it.W + x] : 0;
+    if (t == 0) s_nlist = 0;
+    // the snapshot bytes and the candidates' colours are independent loads:
+    // issue them together
+    for (int k = t; k < RAX * RAY; k += 256) {
+        int x = x0 - 4 + k % RAX, y = y0 - 4 + k / RAX;
+        s_aux[k] = (x >= 0 && y >= 0 && x < it.W && y < it.H) ? aux[(size_t)y * it.W + x] : 0;
+    }
+    for (int k = t; k < RCX * RCY; k += 256) {
+        int x = x0 - 1 + k % RCX, y = y0 - 1 + k / RCX;
+        if (x >= 0 && y >= 0 && x < it.W && y < it.H) s_col[k] = cell_rgba(load_px(im, x, y), it.ch);
+        s_need[k] = 0;
     }
     __syncthreads();
-    // candidate cells: key, colour and the table's coverage rows
+    // heuristic crossing resolution (ambiguous_cross_Kernel) on the snapshot,
+    // for the tile and a halo of 3; the walks read the snapshot in HBM
+    for (int k = t; k < RGX * RGY; k += 256) {
+        int lx = k % RGX, ly = k / RGX;
+        int c = (ly + 1) * RAX + lx + 1;
+        uint32_t own = s_aux[c];
+        if (own & 0xA5u) {  // any diagonal bit (0, 2, 5, 7); off-frame bytes are 0
+            uint32_t nb[8];
+#pragma unroll
+            for (int e = 0; e < 8; e++) nb[e] = s_aux[c + edy(e) * RAX + edx(e)];
+            // snapshot bytes inside the LDS window come from LDS, the rest from HBM
+            auto get = [&](int x, int y) -> uint32_t {
+                int ax = x - (x0 - 4), ay = y - (y0 - 4);
+                if ((unsigned)ax < (unsigned)RAX && (unsigned)ay < (unsigned)RAY) return s_aux[ay * RAX + ax];
+                return aux[(size_t)y * it.W + x];
+            };
+            own &= ~heuristic_clear_mask(get, x0 - 3 + lx, y0 - 3 + ly, own, nb);
+        }
+        s_g[k] = (uint8_t)own;
+    }
+    __syncthreads();
+    for (int oc = t; oc < RTX * RTY; oc += 256) {  // the resolved graph of the tile's own cells
+        int lx = oc % RTX, ly = oc / RTX;
+        int x = x0 + lx, y = y0 + ly;
+        if (x < it.W && y < it.H) gfin[it.g_off + (size_t)y * it.W + x] = s_g[(ly + 3) * RGX + lx + 3];
+    }
+    // candidate cells: key, colour and coverage summary
     for (int k = t; k < RCX * RCY; k += 256) {
         int lx = k % RCX, ly = k / RCX;
         int x = x0 - 1 + lx, y = y0 - 1 + ly;
@@ -259,12 +330,15 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             int c = (ly + 2) * RGX + lx + 2;
             uint32_t key = cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
             s_key[k] = (uint16_t)key;
-            s_col[k] = cell_rgba(load_px(im, x, y), it.ch);
-            const uint32_t* row = mask_table + (size_t)key * MW;
+            s_info[k] = info_table[key];
+            if constexpr (SUB) {
+                const uint32_t* row = mask_table + (size_t)key * MW;
 #pragma unroll
-            for (int r = 0; r < MW; r++) s_mask[k * MW + r] = row[r];
+                for (int r = 0; r < MW; r++) s_mask[k * MW + r] = row[r];
+            }
         } else {
             s_key[k] = kNoCell;
+            s_info[k] = 0;
         }
     }
     if constexpr (SUB) {
@@ -301,19 +375,82 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             uint8_t tri[3 * kMaxPoly];
             int ok;
             int nt = ear_clip(ox, oy, m, tri, &ok);
-            for (int r = 0; r < MW; r++) s_mask[k * MW + r] = coverage_row(ox, oy, tri, nt, S, r);
+            uint32_t rows[MW];
+            for (int r = 0; r < MW; r++) {
+                rows[r] = coverage_row(ox, oy, tri, nt, S, r);
+                s_mask[k * MW + r] = rows[r];
+            }
+            s_info[k] = (uint16_t)coverage_info(rows, S);
         }
     }
     __syncthreads();
 
-    // output: one unit = one output row of one cell (S pixels)
-    constexpr uint32_t FULL = (S >= 32) ? 0xFFFFFFFFu : ((1u << S) - 1u);
     const int OH = it.H * S;
-    for (int u = t; u < RTX * RTY * S; u += 256) {
-        int cx = u % RTX, yr = u / RTX;
-        int cy = yr / S, ky = yr % S;
+    auto store_row = [&](int gx, int Y, const uint32_t* col) {
+        int row = (it.flags & DPX_TOPDOWN) ? (OH - 1 - Y) : Y;
+        uint8_t* dst = out + it.out_off + (size_t)row * it.out_stride + (size_t)gx * S * 4;
+        if constexpr (S % 4 == 0) {
+            if (it.aligned16) {
+#pragma unroll
+                for (int q = 0; q < S; q += 4)
+                    *reinterpret_cast<uint4*>(dst + 4 * q) = make_uint4(col[q], col[q + 1], col[q + 2], col[q + 3]);
+                return;
+            }
+        }
+        if constexpr (S % 2 == 0) {
+            if ((reinterpret_cast<uintptr_t>(dst) & 7) == 0) {
+#pragma unroll
+                for (int q = 0; q < S; q += 2) *reinterpret_cast<uint2*>(dst + 4 * q) = make_uint2(col[q], col[q + 1]);
+                return;
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < S; q++) *reinterpret_cast<uint32_t*>(dst + 4 * q) = col[q];
+    };
+
+    // phase A: each own cell whose S*S pixels are all its own colour writes
+    // them directly; the others go to a list
+    for (int oc = t; oc < RTX * RTY; oc += 256) {
+        int cx = oc % RTX, cy = oc / RTX;
         int gx = x0 + cx, gy = y0 + cy;
-        if (gx >= it.W || gy >= it.H) continue;
+        if (gx < it.W && gy < it.H) {
+            int k = (cy + 1) * RCX + cx + 1;
+            if (simple_cell(s_info[k], s_info[k + RCX - 1], s_info[k + RCX], s_info[k + RCX + 1], s_info[k + 1])) {
+                uint32_t col[S];
+#pragma unroll
+                for (int q = 0; q < S; q++) col[q] = s_col[k];
+#pragma unroll
+                for (int ky = 0; ky < S; ky++) store_row(gx, gy * S + ky, col);
+            } else {
+                s_list[atomicAdd(&s_nlist, 1)] = (uint16_t)oc;
+                if constexpr (!SUB) {
+                    for (int dy = -1; dy <= 1; dy++)
+                        for (int dx = -1; dx <= 1; dx++) s_need[k + dy * RCX + dx] = 1;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if constexpr (!SUB) {
+        // stage the coverage rows of the listed cells' candidates in LDS with
+        // one parallel pass over the table
+        if (s_nlist > 0) {
+            for (int q = t; q < RCX * RCY * MW; q += 256) {
+                int k = q / MW, r = q % MW;
+                if (s_need[k] && s_key[k] != kNoCell) s_mask[q] = mask_table[(size_t)s_key[k] * MW + r];
+            }
+            __syncthreads();
+        }
+    }
+
+    // phase B: one unit = one output row (S pixels) of a listed cell, resolved
+    // against the 3x3 candidates in draw order (highest cell index first)
+    constexpr uint32_t FULL = (S >= 32) ? 0xFFFFFFFFu : ((1u << S) - 1u);
+    const int nunits = s_nlist * S;
+    for (int u = t; u < nunits; u += 256) {
+        int own = s_list[u / S], ky = u % S;
+        int cx = own % RTX, cy = own / RTX;
+        int gx = x0 + cx, gy = y0 + cy;
         uint32_t col[S];
 #pragma unroll
         for (int q = 0; q < S; q++) col[q] = 0xFF000000u;
@@ -339,32 +476,12 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 }
             }
         }
-        int Y = gy * S + ky;
-        int row = (it.flags & DPX_TOPDOWN) ? (OH - 1 - Y) : Y;
-        uint8_t* dst = out + it.out_off + (size_t)row * it.out_stride + (size_t)gx * S * 4;
-        if constexpr (S % 4 == 0) {
-            if (it.aligned16) {
-#pragma unroll
-                for (int q = 0; q < S; q += 4)
-                    *reinterpret_cast<uint4*>(dst + 4 * q) = make_uint4(col[q], col[q + 1], col[q + 2], col[q + 3]);
-                continue;
-            }
-        }
-        if constexpr (S % 2 == 0) {
-            if ((reinterpret_cast<uintptr_t>(dst) & 7) == 0) {
-#pragma unroll
-                for (int q = 0; q < S; q += 2)
-                    *reinterpret_cast<uint2*>(dst + 4 * q) = make_uint2(col[q], col[q + 1]);
-                continue;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < S; q++) *reinterpret_cast<uint32_t*>(dst + 4 * q) = col[q];
+        store_row(gx, gy * S + ky, col);
     }
 }
 
-using RasterFn = void (*)(const DevItem*, const int32_t*, const uint32_t*, int, const uint8_t*, const uint8_t*,
-                          const int8_t*, const uint32_t*, uint8_t*);
+using RasterFn = void (*)(const DevItem*, const int32_t*, const uint32_t*, int, uint32_t, const uint8_t*,
+                          const uint8_t*, uint8_t*, const int8_t*, const uint32_t*, const uint16_t*, uint8_t*);
 
 template <int S>
 RasterFn raster_fn(bool sub)
@@ -403,6 +520,7 @@ struct dpx_context {
     int device = 0;
     int8_t* d_hull = nullptr;
     uint32_t* d_mask[dpx::kMaxScale + 1] = {};
+    uint16_t* d_info[dpx::kMaxScale + 1] = {};
     uint8_t *d_gaux = nullptr, *d_gfin = nullptr;
     size_t g_cap = 0;
     uint8_t* d_desc = nullptr;
@@ -414,7 +532,7 @@ struct dpx_context {
     std::vector<uint8_t> last_desc;
     std::vector<uint64_t> g_offsets;
     std::mutex mu;
-    // kernel timing: 4 events per recorded batch (start, after graph, after heuristic, end)
+    // kernel timing: 3 events per recorded batch (start, after graph, end)
     bool timing = false;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used = 0;
@@ -427,10 +545,12 @@ using namespace dpx;
 
 int ensure_mask(dpx_context* ctx, int S)
 {
-    if (ctx->d_mask[S]) return DPX_OK;
+    if (ctx->d_mask[S] && ctx->d_info[S]) return DPX_OK;
     size_t n = (size_t)kKeys * mask_w(S);
-    DPX_HIP(hipMalloc(&ctx->d_mask[S], n * sizeof(uint32_t)));
+    if (!ctx->d_mask[S]) DPX_HIP(hipMalloc(&ctx->d_mask[S], n * sizeof(uint32_t)));
+    if (!ctx->d_info[S]) DPX_HIP(hipMalloc(&ctx->d_info[S], kKeys * sizeof(uint16_t)));
     hipLaunchKernelGGL(kt_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, ctx->d_hull, S, ctx->d_mask[S]);
+    hipLaunchKernelGGL(kt_info, dim3(kKeys / 256), dim3(256), 0, 0, ctx->d_mask[S], S, ctx->d_info[S]);
     DPX_HIP(hipGetLastError());
     DPX_HIP(hipDeviceSynchronize());
     return DPX_OK;
@@ -444,6 +564,16 @@ struct Group {
 };
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+// Tiles per item when all items of a prefix have the same count, else 0.
+uint32_t uniform_tiles(const std::vector<uint32_t>& prefix)
+{
+    if (prefix.size() < 2) return 0;
+    uint32_t d = prefix[1] - prefix[0];
+    for (size_t k = 2; k < prefix.size(); k++)
+        if (prefix[k] - prefix[k - 1] != d) return 0;
+    return d;
+}
 
 }  // namespace
 
@@ -477,6 +607,7 @@ extern "C" void dpx_context_destroy(dpx_context* ctx)
     (void)hipDeviceSynchronize();
     (void)hipFree(ctx->d_hull);
     for (auto* m : ctx->d_mask) (void)hipFree(m);
+    for (auto* m : ctx->d_info) (void)hipFree(m);
     (void)hipFree(ctx->d_gaux);
     (void)hipFree(ctx->d_gfin);
     (void)hipFree(ctx->d_desc);
@@ -605,23 +736,20 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
     const DevItem* d_items = reinterpret_cast<const DevItem*>(ctx->d_desc + off_items);
     const uint32_t* d_gprefix = reinterpret_cast<const uint32_t*>(ctx->d_desc + off_gprefix);
 
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[3] = {};
     if (ctx->timing) {
-        while (ctx->ev_pool.size() < ctx->ev_used + 4) {
+        while (ctx->ev_pool.size() < ctx->ev_used + 3) {
             hipEvent_t e;
             DPX_HIP(hipEventCreate(&e));
             ctx->ev_pool.push_back(e);
         }
-        for (int q = 0; q < 4; q++) ev[q] = ctx->ev_pool[ctx->ev_used + q];
-        ctx->ev_used += 4;
+        for (int q = 0; q < 3; q++) ev[q] = ctx->ev_pool[ctx->ev_used + q];
+        ctx->ev_used += 3;
         DPX_HIP(hipEventRecord(ev[0], stream));
     }
-    hipLaunchKernelGGL(kf_graph, dim3(gprefix[n_items]), dim3(256), 0, stream, d_items, d_gprefix, n_items, d_in,
-                       ctx->d_gaux);
+    hipLaunchKernelGGL(kf_graph, dim3(gprefix[n_items]), dim3(256), 0, stream, d_items, d_gprefix, n_items,
+                       uniform_tiles(gprefix), d_in, ctx->d_gaux);
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[1], stream));
-    hipLaunchKernelGGL(kf_heur, dim3(gprefix[n_items]), dim3(256), 0, stream, d_items, d_gprefix, n_items,
-                       ctx->d_gaux, ctx->d_gfin);
-    if (ctx->timing) DPX_HIP(hipEventRecord(ev[2], stream));
     int gk = 0;
     for (auto& kv : groups) {
         Group& grp = kv.second;
@@ -630,10 +758,12 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         const int32_t* d_idx = reinterpret_cast<const int32_t*>(ctx->d_desc + goffs[gk].first);
         const uint32_t* d_rp = reinterpret_cast<const uint32_t*>(ctx->d_desc + goffs[gk].second);
         hipLaunchKernelGGL(fn, dim3(grp.prefix.back()), dim3(256), 0, stream, d_items, d_idx, d_rp,
-                           (int)grp.idx.size(), d_in, ctx->d_gfin, ctx->d_hull, ctx->d_mask[grp.scale], d_out);
+                           (int)grp.idx.size(), uniform_tiles(grp.prefix), d_in, ctx->d_gaux, ctx->d_gfin,
+                           ctx->d_hull, ctx->d_mask[grp.scale],
+                           ctx->d_info[grp.scale], d_out);
         gk++;
     }
-    if (ctx->timing) DPX_HIP(hipEventRecord(ev[3], stream));
+    if (ctx->timing) DPX_HIP(hipEventRecord(ev[2], stream));
     DPX_HIP(hipGetLastError());
     return DPX_OK;
 }
@@ -650,16 +780,14 @@ extern "C" int dpx_batch_timing_read(dpx_context* ctx, dpx_batch_times* out)
 {
     DPX_CHECK(ctx && out, DPX_EINVAL, "bad arguments");
     std::lock_guard<std::mutex> lock(ctx->mu);
-    for (size_t b = 0; b + 4 <= ctx->ev_used; b += 4) {
+    for (size_t b = 0; b + 3 <= ctx->ev_used; b += 3) {
         hipEvent_t* ev = &ctx->ev_pool[b];
-        DPX_HIP(hipEventSynchronize(ev[3]));
-        float t01, t12, t23;
+        DPX_HIP(hipEventSynchronize(ev[2]));
+        float t01, t12;
         DPX_HIP(hipEventElapsedTime(&t01, ev[0], ev[1]));
         DPX_HIP(hipEventElapsedTime(&t12, ev[1], ev[2]));
-        DPX_HIP(hipEventElapsedTime(&t23, ev[2], ev[3]));
         ctx->acc.graph_ms += t01;
-        ctx->acc.heuristic_ms += t12;
-        ctx->acc.raster_ms += t23;
+        ctx->acc.raster_ms += t12;
         ctx->acc.batches++;
     }
     ctx->ev_used = 0;

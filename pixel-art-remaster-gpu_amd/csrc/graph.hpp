@@ -16,22 +16,25 @@ __host__ __device__ inline int others(uint32_t node, int edge)
     return __builtin_popcount(node & 0xFFu & ~(1u << edge));
 }
 
-// Offset of the neighbour across edge e in a row-major W-wide graph
-// (calc_index, graph_functions.cu:46-76).
-__host__ __device__ inline int nbr_index(int index, int e, int W) { return index + edy(e) * W + edx(e); }
+// The graph is read through an accessor get(x, y) -> byte, so a kernel can
+// serve the bytes of its own tile from LDS and the rest from HBM.  Only
+// in-frame positions are asked for: walks follow set bits, which never
+// point off the frame.
 
 // calcVal2PathSize (graph_functions.cu:541-573) as a loop: follow valence-2
 // nodes away from `edge`, adding one per step to the shared accumulator
 // while it is <= 30 (so a diagonal's two walks sum to at most 31).
-__host__ __device__ inline void val2_walk(const uint8_t* g, int index, int edge, int* result, int W)
+template <class G>
+__host__ __device__ inline void val2_walk(G get, int x, int y, int edge, int* result)
 {
     for (;;) {
-        uint32_t rest = (uint32_t)g[index] & ~(1u << edge);
+        uint32_t rest = get(x, y) & ~(1u << edge);
         if (*result > 30) return;
         if (__builtin_popcount(rest) != 1) return;
         (*result)++;
         int k = __builtin_ctz(rest);
-        index = nbr_index(index, k, W);
+        x += edx(k);
+        y += edy(k);
         edge = 7 - k;
     }
 }
@@ -39,10 +42,10 @@ __host__ __device__ inline void val2_walk(const uint8_t* g, int index, int edge,
 // processHeuristics2 (graph_functions.cu:761-849) for the 2x2 block whose
 // lower-left pixel is (i,j): i1=(i,j) i2=(i,j+1) i3=(i+1,j) i4=(i+1,j+1).
 // true: remove "/" (i1-i4, bits 2 and 5); false: remove "\" (i2-i3, bits 0 and 7).
-__host__ __device__ inline bool remove_slash(const uint8_t* g, int W, int i, int j)
+template <class G>
+__host__ __device__ inline bool remove_slash(G get, int i, int j)
 {
-    int i1 = j * W + i, i2 = i1 + W, i3 = i1 + 1, i4 = i1 + W + 1;
-    uint32_t n1 = g[i1], n2 = g[i2], n3 = g[i3], n4 = g[i4];
+    uint32_t n1 = get(i, j), n2 = get(i, j + 1), n3 = get(i + 1, j), n4 = get(i + 1, j + 1);
     int o1 = others(n1, 2), o4 = others(n4, 5), o3 = others(n3, 0), o2 = others(n2, 7);
     // curves (checkValence2Edge 421-453)
     if (o1 == 1 && o4 == 1) return false;
@@ -53,10 +56,10 @@ __host__ __device__ inline bool remove_slash(const uint8_t* g, int W, int i, int
     if (v3 || (v2 && !v1 && !v4)) return true;
     // curve lengths; ties remove "/"
     int seg = 0, seg2 = 0;
-    val2_walk(g, i1, 2, &seg, W);
-    val2_walk(g, i4, 5, &seg, W);
-    val2_walk(g, i3, 0, &seg2, W);
-    val2_walk(g, i2, 7, &seg2, W);
+    val2_walk(get, i, j, 2, &seg);
+    val2_walk(get, i + 1, j + 1, 5, &seg);
+    val2_walk(get, i + 1, j, 0, &seg2);
+    val2_walk(get, i, j + 1, 7, &seg2);
     return !(seg2 < seg);
 }
 
@@ -89,22 +92,22 @@ __host__ __device__ inline uint32_t trivial_clear_mask(uint32_t c, const uint32_
 // Diagonal bits the pixel (i,j) loses in ambiguous_cross_Kernel
 // (crossCheck_Heuristics graph_functions.cu:1433-1518), reading the snapshot
 // `aux`.  nb[e] = snapshot byte of neighbour e (0 off-frame); c = own byte.
-__host__ __device__ inline uint32_t heuristic_clear_mask(const uint8_t* aux, int W, int i, int j, uint32_t c,
-                                                         const uint32_t* nb)
+template <class G>
+__host__ __device__ inline uint32_t heuristic_clear_mask(G get, int i, int j, uint32_t c, const uint32_t* nb)
 {
     uint32_t clear = 0;
     // up-left block, origin (i-1,j): crossed if left has bit 2, up-left bit 7, self bit 0, up bit 5
     if (bit(nb[3], 2) && bit(nb[0], 7) && bit(c, 0) && bit(nb[1], 5))
-        if (!remove_slash(aux, W, i - 1, j)) clear |= 1u << 0;
+        if (!remove_slash(get, i - 1, j)) clear |= 1u << 0;
     // down-left block, origin (i-1,j-1)
     if (bit(nb[5], 2) && bit(nb[3], 7) && bit(nb[6], 0) && bit(c, 5))
-        if (remove_slash(aux, W, i - 1, j - 1)) clear |= 1u << 5;
+        if (remove_slash(get, i - 1, j - 1)) clear |= 1u << 5;
     // down-right block, origin (i,j-1)
     if (bit(nb[6], 2) && bit(c, 7) && bit(nb[7], 0) && bit(nb[4], 5))
-        if (!remove_slash(aux, W, i, j - 1)) clear |= 1u << 7;
+        if (!remove_slash(get, i, j - 1)) clear |= 1u << 7;
     // up-right block, origin (i,j)
     if (bit(c, 2) && bit(nb[1], 7) && bit(nb[4], 0) && bit(nb[2], 5))
-        if (remove_slash(aux, W, i, j)) clear |= 1u << 2;
+        if (remove_slash(get, i, j)) clear |= 1u << 2;
     return clear;
 }
 

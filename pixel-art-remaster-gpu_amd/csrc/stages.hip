@@ -72,7 +72,8 @@ __global__ void k_heuristic(uint8_t* __restrict__ graph, const uint8_t* __restri
     gather_nb(aux, W, H, i, j, nb);
     size_t idx = (size_t)j * W + i;
     uint32_t c = aux[idx];
-    graph[idx] = (uint8_t)(c & ~heuristic_clear_mask(aux, W, i, j, c, nb));
+    auto get = [=](int x, int y) -> uint32_t { return aux[(size_t)y * W + x]; };
+    graph[idx] = (uint8_t)(c & ~heuristic_clear_mask(get, i, j, c, nb));
 }
 
 // cells_Kernel (kernel.cu:192-213): hull points in 45-slot cells.  Slots past

@@ -39,7 +39,7 @@ class Item(ctypes.Structure):
 
 
 class BatchTimes(ctypes.Structure):
-    _fields_ = [("graph_ms", c_double), ("heuristic_ms", c_double), ("raster_ms", c_double), ("batches", c_uint64)]
+    _fields_ = [("graph_ms", c_double), ("raster_ms", c_double), ("batches", c_uint64)]
 
 
 class ServerStats(ctypes.Structure):

@@ -45,7 +45,8 @@ struct DevItem {
     uint32_t flags;
     int32_t out_stride, scale, subdivide;
     int32_t aligned16;  // output rows 16-byte aligned
-    int32_t pad[3];
+    int32_t graph_w4;   // graph rows 4-byte aligned (W % 4 == 0)
+    int32_t pad[2];
 };
 
 // ---------------------------------------------------------------------------
@@ -169,77 +170,92 @@ __device__ inline void block_item(const uint32_t* __restrict__ prefix, int n, ui
 // ---------------------------------------------------------------------------
 // kf_graph: similarity graph + trivial crossing removal
 // ---------------------------------------------------------------------------
-constexpr int GTX = 64, GTY = 16;
-constexpr int GLX = GTX + 4, GLY = GTY + 4;  // YUV tile, halo 2
-constexpr int GBX = GTX + 2, GBY = GTY + 2;  // bit tile, halo 1
+// Three passes over LDS:
+//   1. lane words (pixel class + YUV fields, see lane_ref) of the tile and a
+//      halo: x0-1 .. x0+64, y0-2 .. y0+33;
+//   2. per position of x0-1 .. x0+63, y0-1 .. y0+32, the four edges it owns
+//      (up, up-right, right, down-right: bits 1, 2, 4, 7; similarity is
+//      symmetric, so each edge is tested once) and in bit 0 whether the 2x2
+//      block whose lower-left pixel it is has all four orthogonal edges
+//      ("full"), which is crossCheck_4's condition for removing both
+//      diagonals;
+//   3. per four pixels of a row at once (one byte each in a 32-bit word):
+//      the other four edges from the left, down, down-left and up-left
+//      neighbours' bytes and the trivial removal from the full flags of the
+//      four blocks around the pixel.
+constexpr int GTX = 64, GTY = 32;
+constexpr int GLX = GTX + 2, GLY = GTY + 4;  // lane words
+constexpr int GHX = GTX + 1, GHY = GTY + 2;  // positions of pass 2
+constexpr int GHP = GTX + 4;                 // byte pitch of pass 2's rows: column c is x = x0 - 4 + c
 
 __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix,
                                                 int n_items, uint32_t uniform, const uint8_t* __restrict__ in,
                                                 uint8_t* __restrict__ gaux)
 {
-    __shared__ uint32_t s_yuv[GLX * GLY];
-    __shared__ uint8_t s_cls[GLX * GLY];   // 0 opaque, 1 transparent, 2 outside the frame
-    __shared__ uint8_t s_half[GLX * GLY];  // bits 1, 2, 4, 7 (up, up-right, right, down-right)
-    __shared__ uint8_t s_bits[GBX * GBY];
+    __shared__ uint32_t s_lane[GLX * GLY];
+    __shared__ uint32_t s_h32[GHY * GHP / 4];
+    uint8_t* s_h = reinterpret_cast<uint8_t*>(s_h32);
     const int t = threadIdx.x;
     int ii, tile;
     block_item(prefix, n_items, uniform, blockIdx.x, &ii, &tile);
     const DevItem it = items[ii];
     const int tiles_x = (it.W + GTX - 1) / GTX;
     const int x0 = (tile % tiles_x) * GTX, y0 = (tile / tiles_x) * GTY;
-    ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags };
 
+    // pass 1.  Byte offsets within an item fit 31 bits (checked on the host).
+    const uint8_t* base = in + it.in_off;
+    const bool rgb = it.flags & DPX_RGB, topdown = it.flags & DPX_TOPDOWN;
+    const bool alpha = (it.flags & DPX_ALPHA) && it.ch == 4;
+    const uint32_t o0 = rgb ? 2 : 0, o2 = 2 - o0;
     for (int k = t; k < GLX * GLY; k += 256) {
-        int x = x0 - 2 + k % GLX, y = y0 - 2 + k / GLX;
-        if (x >= 0 && y >= 0 && x < it.W && y < it.H) {
-            Yuv v = yuv_of(load_px(im, x, y), it.flags, it.ch);
-            s_yuv[k] = v.w;
-            s_cls[k] = (uint8_t)v.transparent;
-        } else {
-            s_cls[k] = 2;
+        int x = x0 - 1 + k % GLX, y = y0 - 2 + k / GLX;
+        uint32_t lw = 0;
+        if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
+            int row = topdown ? it.H - 1 - y : y;
+            uint32_t off = __umul24(row, it.stride) + __umul24(x, it.ch);  // operands < 2^24
+            int c0 = base[off + o0], c1 = base[off + 1], c2 = base[off + o2];
+            lw = yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque;
+            if (alpha && base[off + 3] == 0) lw = kLaneTransparent;
         }
+        s_lane[k] = lw;
     }
     __syncthreads();
-    // Similarity is symmetric: test the four directions up, up-right, right
-    // and down-right once per pixel, and take the other four from the
-    // neighbours' bits below.
-    for (int k = t; k < GLX * GLY; k += 256) {
-        int lx = k % GLX, ly = k / GLX;
-        uint32_t bits = 0;
-        if (s_cls[k] != 2) {
-            Yuv a{ s_yuv[k], s_cls[k] };
-            auto test = [&](int q, int e) {
-                if (s_cls[q] != 2 && similar(a, Yuv{ s_yuv[q], s_cls[q] })) bits |= 1u << e;
-            };
-            bool up = ly + 1 < GLY, rt = lx + 1 < GLX;
-            if (up) test(k + GLX, 1);
-            if (up && rt) test(k + GLX + 1, 2);
-            if (rt) test(k + 1, 4);
-            if (rt && ly >= 1) test(k - GLX + 1, 7);
-        }
-        s_half[k] = (uint8_t)bits;
+    // pass 2
+    for (int k = t; k < GHX * GHY; k += 256) {
+        int cx = k % GHX, cy = k / GHX;
+        int c = (cy + 1) * GLX + cx;  // lane word of position (x0-1+cx, y0-1+cy)
+        uint32_t up = s_lane[c + GLX], ur = s_lane[c + GLX + 1], rt = s_lane[c + 1], dr = s_lane[c - GLX + 1];
+        LaneRef a = lane_ref(s_lane[c]), b = lane_ref(ur);
+        uint32_t u = lane_similar(a, up), r = lane_similar(a, rt);
+        uint32_t h = u << 1 | lane_similar(a, ur) << 2 | r << 4 | lane_similar(a, dr) << 7;
+        h |= u & r & lane_similar(b, up) & lane_similar(b, rt);
+        s_h[cy * GHP + cx + 3] = (uint8_t)h;
     }
     __syncthreads();
-    for (int k = t; k < GBX * GBY; k += 256) {
-        int c = (k / GBX + 1) * GLX + k % GBX + 1;
-        uint32_t bits = s_half[c];
-        bits |= (((uint32_t)s_half[c - 1] >> 4) & 1u) << 3;        // left: its right bit
-        bits |= (((uint32_t)s_half[c - GLX] >> 1) & 1u) << 6;      // down: its up bit
-        bits |= (((uint32_t)s_half[c - GLX - 1] >> 2) & 1u) << 5;  // down-left: its up-right bit
-        bits |= (((uint32_t)s_half[c + GLX - 1] >> 7) & 1u) << 0;  // up-left: its down-right bit
-        s_bits[k] = (uint8_t)bits;
-    }
-    __syncthreads();
+    // pass 3
     uint8_t* g = gaux + it.g_off;
-    for (int k = t; k < GTX * GTY; k += 256) {
-        int lx = k % GTX, ly = k / GTX;
-        int x = x0 + lx, y = y0 + ly;
+    for (int k = t; k < GTX / 4 * GTY; k += 256) {
+        int q = k % (GTX / 4), ly = k / (GTX / 4);
+        int x = x0 + 4 * q, y = y0 + ly;
         if (x >= it.W || y >= it.H) continue;
-        int c = (ly + 1) * GBX + lx + 1;
-        uint32_t own = s_bits[c], nb[8];
-#pragma unroll
-        for (int e = 0; e < 8; e++) nb[e] = s_bits[c + edy(e) * GBX + edx(e)];
-        g[(size_t)y * it.W + x] = (uint8_t)(own & ~trivial_clear_mask(own, nb));
+        const uint32_t* dn = s_h32 + ly * (GHP / 4) + q;  // row y-1; word q+1 holds x .. x+3
+        const uint32_t* md = dn + GHP / 4;
+        const uint32_t* up = md + GHP / 4;
+        uint32_t own = md[1], left = __builtin_amdgcn_alignbyte(md[1], md[0], 3);
+        uint32_t down = dn[1], dl = __builtin_amdgcn_alignbyte(dn[1], dn[0], 3);
+        uint32_t ul = __builtin_amdgcn_alignbyte(up[1], up[0], 3);
+        uint32_t edges = (own & 0x96969696u) | ((left & 0x10101010u) >> 1) | ((down & 0x02020202u) << 5) |
+                         ((dl & 0x04040404u) << 3) | ((ul & 0x80808080u) >> 7);
+        uint32_t clear = ((own & 0x01010101u) << 2) | (left & 0x01010101u) | ((dl & 0x01010101u) << 5) |
+                         ((down & 0x01010101u) << 7);
+        uint32_t v = edges & ~clear;
+        uint8_t* dst = g + (uint32_t)(y * it.W + x);
+        if (it.graph_w4) {
+            *reinterpret_cast<uint32_t*>(dst) = v;
+        } else {
+            int n = min(4, it.W - x);
+            for (int b = 0; b < n; b++) dst[b] = (uint8_t)(v >> (8 * b));
+        }
     }
 }
 
@@ -248,11 +264,21 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
 // gather raster of a 32x8-cell tile
 // ---------------------------------------------------------------------------
 constexpr int RTX = 32, RTY = 8;
-constexpr int RAX = RTX + 8, RAY = RTY + 8;  // snapshot graph bytes, halo 4
-constexpr int RGX = RTX + 6, RGY = RTY + 6;  // final graph bytes, halo 3
 constexpr int RHX = RTX + 4, RHY = RTY + 4;  // hulls, halo 2 (subdivision)
 constexpr int RCX = RTX + 2, RCY = RTY + 2;  // candidate cells, halo 1
 constexpr uint16_t kNoCell = 0xFFFF;
+
+// LDS windows of the raster tile.  Candidate cells (halo 1) need the resolved
+// bytes of their left and right neighbours for their keys: resolved bytes at
+// halo (2, 1).  With subdivision the hulls at halo 2 are needed as well:
+// resolved bytes at halo (3, 2).  Resolving a byte reads the snapshot bytes
+// around it: one more in each direction.
+template <bool SUB>
+struct RasterWin {
+    static constexpr int GHX = SUB ? 3 : 2, GHY = SUB ? 2 : 1;  // resolved bytes
+    static constexpr int GX = RTX + 2 * GHX, GY = RTY + 2 * GHY;
+    static constexpr int AX = GX + 2, AY = GY + 2;  // snapshot bytes
+};
 
 template <int S, bool SUB>
 __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const int32_t* __restrict__ group,
@@ -262,16 +288,17 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                                                  const uint32_t* __restrict__ mask_table,
                                                  const uint16_t* __restrict__ info_table, uint8_t* __restrict__ out)
 {
+    using Win = RasterWin<SUB>;
+    constexpr int GX = Win::GX, GY = Win::GY, AX = Win::AX, AY = Win::AY, GHX = Win::GHX, GHY = Win::GHY;
     constexpr int MW = mask_w(S), KMIN = mask_kmin(S);
-    __shared__ uint8_t s_aux[RAX * RAY];
-    __shared__ uint8_t s_g[RGX * RGY];
+    __shared__ uint8_t s_aux[AX * AY];
+    __shared__ uint8_t s_g[GX * GY];
     __shared__ uint16_t s_key[RCX * RCY];
     __shared__ uint16_t s_info[RCX * RCY];
     __shared__ uint32_t s_col[RCX * RCY];
-    __shared__ uint32_t s_mask[RCX * RCY * MW];  // coverage rows of the candidates that need them
+    __shared__ uint32_t s_mask[SUB ? RCX * RCY * MW : 1];  // coverage rows of the smoothed candidates
     __shared__ int8_t s_hull[SUB ? RHX * RHY * 17 : 1];
     __shared__ uint16_t s_list[RTX * RTY];  // own cells that need the candidate search
-    __shared__ uint8_t s_need[RCX * RCY];   // candidates of listed cells
     __shared__ int s_nlist;
 
     const int t = threadIdx.x;
@@ -282,69 +309,67 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     const int x0 = (tile % tiles_x) * RTX, y0 = (tile / tiles_x) * RTY;
     ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags };
     const uint8_t* aux = gaux + it.g_off;
+    const int ax0 = x0 - GHX - 1, ay0 = y0 - GHY - 1;  // origin of the snapshot window
 
     if (t == 0) s_nlist = 0;
     // the snapshot bytes and the candidates' colours are independent loads:
     // issue them together
-    for (int k = t; k < RAX * RAY; k += 256) {
-        int x = x0 - 4 + k % RAX, y = y0 - 4 + k / RAX;
-        s_aux[k] = (x >= 0 && y >= 0 && x < it.W && y < it.H) ? aux[(size_t)y * it.W + x] : 0;
+    for (int k = t; k < AX * AY; k += 256) {
+        int x = ax0 + k % AX, y = ay0 + k / AX;
+        bool in_frame = (unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H;
+        s_aux[k] = in_frame ? aux[(uint32_t)(y * it.W + x)] : 0;
     }
     for (int k = t; k < RCX * RCY; k += 256) {
         int x = x0 - 1 + k % RCX, y = y0 - 1 + k / RCX;
-        if (x >= 0 && y >= 0 && x < it.W && y < it.H) s_col[k] = cell_rgba(load_px(im, x, y), it.ch);
-        s_need[k] = 0;
+        if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) s_col[k] = cell_rgba(load_px(im, x, y), it.ch);
     }
     __syncthreads();
-    // heuristic crossing resolution (ambiguous_cross_Kernel) on the snapshot,
-    // for the tile and a halo of 3; the walks read the snapshot in HBM
-    for (int k = t; k < RGX * RGY; k += 256) {
-        int lx = k % RGX, ly = k / RGX;
-        int c = (ly + 1) * RAX + lx + 1;
+    // heuristic crossing resolution (ambiguous_cross_Kernel) on the snapshot;
+    // the walks read the snapshot in LDS where they can, else in HBM
+    for (int k = t; k < GX * GY; k += 256) {
+        int lx = k % GX, ly = k / GX;
+        int c = (ly + 1) * AX + lx + 1;
         uint32_t own = s_aux[c];
         if (own & 0xA5u) {  // any diagonal bit (0, 2, 5, 7); off-frame bytes are 0
             uint32_t nb[8];
 #pragma unroll
-            for (int e = 0; e < 8; e++) nb[e] = s_aux[c + edy(e) * RAX + edx(e)];
-            // snapshot bytes inside the LDS window come from LDS, the rest from HBM
+            for (int e = 0; e < 8; e++) nb[e] = s_aux[c + edy(e) * AX + edx(e)];
             auto get = [&](int x, int y) -> uint32_t {
-                int ax = x - (x0 - 4), ay = y - (y0 - 4);
-                if ((unsigned)ax < (unsigned)RAX && (unsigned)ay < (unsigned)RAY) return s_aux[ay * RAX + ax];
-                return aux[(size_t)y * it.W + x];
+                int wx = x - ax0, wy = y - ay0;
+                if ((unsigned)wx < (unsigned)AX && (unsigned)wy < (unsigned)AY) return s_aux[wy * AX + wx];
+                return aux[(uint32_t)(y * it.W + x)];
             };
-            own &= ~heuristic_clear_mask(get, x0 - 3 + lx, y0 - 3 + ly, own, nb);
+            own &= ~heuristic_clear_mask(get, ax0 + 1 + lx, ay0 + 1 + ly, own, nb);
         }
         s_g[k] = (uint8_t)own;
+        // the resolved graph of the tile's own cells
+        int cx = lx - GHX, cy = ly - GHY;
+        if ((unsigned)cx < (unsigned)RTX && (unsigned)cy < (unsigned)RTY && x0 + cx < it.W && y0 + cy < it.H)
+            gfin[it.g_off + (uint32_t)((y0 + cy) * it.W + x0 + cx)] = (uint8_t)own;
     }
     __syncthreads();
-    for (int oc = t; oc < RTX * RTY; oc += 256) {  // the resolved graph of the tile's own cells
-        int lx = oc % RTX, ly = oc / RTX;
-        int x = x0 + lx, y = y0 + ly;
-        if (x < it.W && y < it.H) gfin[it.g_off + (size_t)y * it.W + x] = s_g[(ly + 3) * RGX + lx + 3];
-    }
-    // candidate cells: key, colour and coverage summary
+    // candidate cells: key and coverage summary
     for (int k = t; k < RCX * RCY; k += 256) {
         int lx = k % RCX, ly = k / RCX;
         int x = x0 - 1 + lx, y = y0 - 1 + ly;
-        if (x >= 0 && y >= 0 && x < it.W && y < it.H) {
-            int c = (ly + 2) * RGX + lx + 2;
-            uint32_t key = cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
-            s_key[k] = (uint16_t)key;
-            s_info[k] = info_table[key];
+        uint16_t key = kNoCell, info = 0;
+        if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
+            int c = (ly - 1 + GHY) * GX + lx - 1 + GHX;
+            key = (uint16_t)cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
+            info = info_table[key];
             if constexpr (SUB) {
                 const uint32_t* row = mask_table + (size_t)key * MW;
 #pragma unroll
                 for (int r = 0; r < MW; r++) s_mask[k * MW + r] = row[r];
             }
-        } else {
-            s_key[k] = kNoCell;
-            s_info[k] = 0;
         }
+        s_key[k] = key;
+        s_info[k] = info;
     }
     if constexpr (SUB) {
         for (int k = t; k < RHX * RHY; k += 256) {
             int lx = k % RHX, ly = k / RHX;
-            int c = (ly + 1) * RGX + lx + 1;
+            int c = (ly - 2 + GHY) * GX + lx - 2 + GHX;
             uint32_t key = cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
             const int8_t* e = hull_table + (size_t)key * kHullBytes;
 #pragma unroll
@@ -357,7 +382,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             if (s_key[k] == kNoCell) continue;
             int lx = k % RCX, ly = k / RCX;
             int x = x0 - 1 + lx, y = y0 - 1 + ly;
-            uint32_t node = s_g[(ly + 2) * RGX + lx + 2];
+            uint32_t node = s_g[(ly - 1 + GHY) * GX + lx - 1 + GHX];
             if (node == 90) continue;  // internal node keeps its hull (kernel.cu:231)
             int hk = (ly + 1) * RHX + lx + 1;
             int n = s_hull[hk * 17];
@@ -423,52 +448,49 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 for (int ky = 0; ky < S; ky++) store_row(gx, gy * S + ky, col);
             } else {
                 s_list[atomicAdd(&s_nlist, 1)] = (uint16_t)oc;
-                if constexpr (!SUB) {
-                    for (int dy = -1; dy <= 1; dy++)
-                        for (int dx = -1; dx <= 1; dx++) s_need[k + dy * RCX + dx] = 1;
-                }
             }
         }
     }
     __syncthreads();
-    if constexpr (!SUB) {
-        // stage the coverage rows of the listed cells' candidates in LDS with
-        // one parallel pass over the table
-        if (s_nlist > 0) {
-            for (int q = t; q < RCX * RCY * MW; q += 256) {
-                int k = q / MW, r = q % MW;
-                if (s_need[k] && s_key[k] != kNoCell) s_mask[q] = mask_table[(size_t)s_key[k] * MW + r];
-            }
-            __syncthreads();
-        }
-    }
 
     // phase B: one unit = one output row (S pixels) of a listed cell, resolved
-    // against the 3x3 candidates in draw order (highest cell index first)
+    // against the 3x3 candidates in draw order (highest cell index first).
+    // The candidates' coverage rows are all fetched before they are combined.
     constexpr uint32_t FULL = (S >= 32) ? 0xFFFFFFFFu : ((1u << S) - 1u);
     const int nunits = s_nlist * S;
     for (int u = t; u < nunits; u += 256) {
         int own = s_list[u / S], ky = u % S;
         int cx = own % RTX, cy = own / RTX;
         int gx = x0 + cx, gy = y0 + cy;
+        uint32_t rows[9];
+#pragma unroll
+        for (int dy = 1; dy >= -1; dy--) {
+#pragma unroll
+            for (int dx = 1; dx >= -1; dx--) {
+                int k = (cy + 1 + dy) * RCX + (cx + 1 + dx);
+                int r = ky - dy * S - KMIN;
+                uint32_t row = 0;
+                if (r >= 0 && r < MW && s_key[k] != kNoCell) {
+                    if constexpr (SUB) row = s_mask[k * MW + r];
+                    else row = mask_table[(uint32_t)s_key[k] * MW + r];
+                }
+                rows[(1 - dy) * 3 + (1 - dx)] = row;
+            }
+        }
         uint32_t col[S];
 #pragma unroll
         for (int q = 0; q < S; q++) col[q] = 0xFF000000u;
         uint32_t assigned = 0;
 #pragma unroll
         for (int dy = 1; dy >= -1; dy--) {
-            int r = ky - dy * S - KMIN;
-            if (r < 0 || r >= MW) continue;
 #pragma unroll
             for (int dx = 1; dx >= -1; dx--) {
-                int k = (cy + 1 + dy) * RCX + (cx + 1 + dx);
-                if (s_key[k] == kNoCell) continue;
-                uint32_t row = s_mask[k * MW + r];
+                uint32_t row = rows[(1 - dy) * 3 + (1 - dx)];
                 int sh = -dx * S - KMIN;
                 uint32_t bits = (sh >= 0 ? (row >> sh) : (row << -sh)) & FULL;
                 uint32_t nb = bits & ~assigned;
                 if (nb) {
-                    uint32_t c = s_col[k];
+                    uint32_t c = s_col[(cy + 1 + dy) * RCX + (cx + 1 + dx)];
 #pragma unroll
                     for (int q = 0; q < S; q++)
                         if ((nb >> q) & 1u) col[q] = c;
@@ -645,6 +667,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
                   "item %d: bad size %dx%d", k, a.width, a.height);
         DPX_CHECK(a.channels == 3 || a.channels == 4, DPX_EINVAL, "item %d: channels %d", k, a.channels);
         DPX_CHECK(a.stride >= a.width * a.channels, DPX_EINVAL, "item %d: stride %d", k, a.stride);
+        DPX_CHECK((int64_t)a.stride * a.height < (int64_t)1 << 31, DPX_EINVAL, "item %d: input larger than 2 GiB", k);
         DPX_CHECK(a.scale >= 1 && a.scale <= kMaxScale, DPX_EINVAL, "item %d: scale %d outside 1..%d", k, a.scale,
                   kMaxScale);
         DPX_CHECK(a.out_stride >= 4 * a.width * a.scale && a.out_stride % 4 == 0 && a.out_offset % 4 == 0, DPX_EINVAL,
@@ -665,6 +688,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         d.subdivide = a.subdivide ? 1 : 0;
         d.aligned16 = (a.out_stride % 16 == 0) && (a.out_offset % 16 == 0) &&
                       (reinterpret_cast<uintptr_t>(d_out) % 16 == 0);
+        d.graph_w4 = a.width % 4 == 0;
         ctx->g_offsets[k] = goff;
         goff += align_up((size_t)a.width * a.height, 256);
         uint32_t gt = (uint32_t)(((a.width + GTX - 1) / GTX) * ((a.height + GTY - 1) / GTY));

@@ -41,16 +41,80 @@ __host__ __device__ inline Px load_px(const ImgView& im, int i, int j)
 // byte 0, b is byte 2.  Luma in IEEE double, (0.299 r + 0.587 g) + 0.114 b
 // with no contraction; U and V truncated float products that may be negative
 // and borrow into the packed word as the reference's int shifts do.
-__host__ __device__ inline uint32_t yuv_pack(int c0, int c1, int c2)
+//
+// Luma without floating point where it can be: with n = 299 c0 + 587 c1 +
+// 114 c2 the real value is n/1000, and the double expression is within
+// 3e-13 of it.  Unless 1000 divides n, n/1000 is at least 0.001 from an
+// integer, so the truncated double equals n / 1000.  Only for the exact
+// multiples (16,774 of the 2^24 colours, every grey among them) can the
+// double land just below the integer; those take the double expression.
+// The quotient comes from single precision: n * 0.001f + 0.0005f is within
+// 5e-5 of n/1000 + 0.0005, and n/1000 + 0.0005 is at least 0.0005 from an
+// integer for every n, so it truncates to floor(n / 1000).
+__host__ __device__ inline uint32_t mul24(uint32_t a, uint32_t b)  // a, b < 2^24
+{
+#ifdef __HIP_DEVICE_COMPILE__
+    return __umul24(a, b);
+#else
+    return a * b;
+#endif
+}
+
+__host__ __device__ inline int luma(int c0, int c1, int c2)
 {
 #pragma clang fp contract(off)
+    int n = 299 * c0 + 587 * c1 + 114 * c2;
+    int q = (int)((float)n * 0.001f + 0.0005f);
+    if ((int)mul24((uint32_t)q, 1000u) != n) return q;
     double yd = 0.299 * (double)c0;
     yd = yd + 0.587 * (double)c1;
     yd = yd + 0.114 * (double)c2;
-    int y = (int)yd;
+    return (int)yd;
+}
+
+__host__ __device__ inline uint32_t yuv_pack(int c0, int c1, int c2)
+{
+    int y = luma(c0, c1, c2);
     int u = (int)((float)(c2 - y) * 0.492f);
     int v = (int)((float)(c0 - y) * 0.877f);
     return (uint32_t)y * 65536u + (uint32_t)u * 256u + (uint32_t)v;
+}
+
+// Lane word of a pixel for a branch-free similarity test: the three masked
+// fields of its packed word in 10-bit lanes (Y at bit 20, U at bit 10, V at
+// bit 0) and a class in bits 30-31: 1 opaque, 2 transparent (lanes 0), 0 for
+// a position outside the frame.
+constexpr uint32_t kLaneOpaque = 1u << 30, kLaneTransparent = 2u << 30;
+
+__host__ __device__ inline uint32_t yuv_lanes(uint32_t w)
+{
+    return ((w >> 16) & 0xFFu) << 20 | ((w >> 8) & 0xFFu) << 10 | (w & 0xFFu);
+}
+
+// |dY| <= 5, |dU| <= 7, |dV| <= 6 on the fields, all three lanes at once:
+// L = a - b + 256 + t stays inside its lane (7..517, so no lane borrows from
+// the next and the class bits subtract exactly, mod 4); a field is within
+// threshold iff 256 <= L <= 256 + 2t, i.e. bit 8 of L is set and M = L + 255
+// - 2t does not reach bit 9.  Equal classes leave bits 30-31 of L (and M)
+// clear.  A pair with one end outside the frame never passes; two ends
+// outside the frame do, and callers never ask for such a pair.
+struct LaneRef {
+    uint32_t not_s;  // ~(a + B1): ~L = b + not_s
+    uint32_t m;      // a + B1 + B2: M = m - b
+};
+
+__host__ __device__ inline LaneRef lane_ref(uint32_t la)
+{
+    const uint32_t B1 = (256u + 5) << 20 | (256u + 7) << 10 | (256u + 6);
+    const uint32_t B2 = (255u - 10) << 20 | (255u - 14) << 10 | (255u - 12);
+    uint32_t s = la + B1;
+    return LaneRef{ ~s, s + B2 };
+}
+
+__host__ __device__ inline uint32_t lane_similar(const LaneRef& a, uint32_t lb)
+{
+    uint32_t bad = (0x10040100u & (a.not_s + lb)) | ((a.m - lb) & 0xE0080200u);
+    return bad == 0 ? 1u : 0u;
 }
 
 __host__ __device__ inline int iabs(int a) { return a < 0 ? -a : a; }

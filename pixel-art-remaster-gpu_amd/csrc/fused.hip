@@ -32,6 +32,9 @@
 
 namespace dpx {
 
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+
 constexpr int kMaxScale = 15;
 constexpr int kKeys = 4096;
 constexpr int kHullBytes = 32;  // n, x[8], y[8] (quarter units), padding
@@ -410,6 +413,9 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     }
     __syncthreads();
 
+    // Output pixels are written once and not read back by the kernels:
+    // non-temporal stores keep them from displacing the graph and table lines
+    // the other tiles' loads need in the caches.
     const int OH = it.H * S;
     auto store_row = [&](int gx, int Y, const uint32_t* col) {
         int row = (it.flags & DPX_TOPDOWN) ? (OH - 1 - Y) : Y;
@@ -417,20 +423,25 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         if constexpr (S % 4 == 0) {
             if (it.aligned16) {
 #pragma unroll
-                for (int q = 0; q < S; q += 4)
-                    *reinterpret_cast<uint4*>(dst + 4 * q) = make_uint4(col[q], col[q + 1], col[q + 2], col[q + 3]);
+                for (int q = 0; q < S; q += 4) {
+                    u32x4 v = { col[q], col[q + 1], col[q + 2], col[q + 3] };
+                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(dst + 4 * q));
+                }
                 return;
             }
         }
         if constexpr (S % 2 == 0) {
             if ((reinterpret_cast<uintptr_t>(dst) & 7) == 0) {
 #pragma unroll
-                for (int q = 0; q < S; q += 2) *reinterpret_cast<uint2*>(dst + 4 * q) = make_uint2(col[q], col[q + 1]);
+                for (int q = 0; q < S; q += 2) {
+                    u32x2 v = { col[q], col[q + 1] };
+                    __builtin_nontemporal_store(v, reinterpret_cast<u32x2*>(dst + 4 * q));
+                }
                 return;
             }
         }
 #pragma unroll
-        for (int q = 0; q < S; q++) *reinterpret_cast<uint32_t*>(dst + 4 * q) = col[q];
+        for (int q = 0; q < S; q++) __builtin_nontemporal_store(col[q], reinterpret_cast<uint32_t*>(dst + 4 * q));
     };
 
     // phase A: each own cell whose S*S pixels are all its own colour writes

@@ -271,16 +271,19 @@ constexpr int RHX = RTX + 4, RHY = RTY + 4;  // hulls, halo 2 (subdivision)
 constexpr int RCX = RTX + 2, RCY = RTY + 2;  // candidate cells, halo 1
 constexpr uint16_t kNoCell = 0xFFFF;
 
-// LDS windows of the raster tile.  Candidate cells (halo 1) need the resolved
-// bytes of their left and right neighbours for their keys: resolved bytes at
-// halo (2, 1).  With subdivision the hulls at halo 2 are needed as well:
-// resolved bytes at halo (3, 2).  Resolving a byte reads the snapshot bytes
-// around it: one more in each direction.
+// LDS windows of the raster tile hold graph bytes in rows of RWW 32-bit
+// words; word w of a row holds x = x0 - 8 + 4w .. x0 - 5 + 4w.  Words 1..10
+// (x0 - 4 .. x0 + 35) carry data, words 0 and 11 stay zero.  Candidate cells
+// (halo 1) need the resolved bytes of their left and right neighbours for
+// their keys: resolved rows at halo 1; with subdivision the hulls at halo 2
+// are needed as well: resolved rows at halo 2.  Resolving a byte reads the
+// snapshot rows above and below it.
+constexpr int RWW = 12, RWB = 4 * RWW;
+static_assert(RTX * RTY == 256, "one own cell per thread of a 256-thread block");
 template <bool SUB>
 struct RasterWin {
-    static constexpr int GHX = SUB ? 3 : 2, GHY = SUB ? 2 : 1;  // resolved bytes
-    static constexpr int GX = RTX + 2 * GHX, GY = RTY + 2 * GHY;
-    static constexpr int AX = GX + 2, AY = GY + 2;  // snapshot bytes
+    static constexpr int GHY = SUB ? 2 : 1;               // resolved rows y0 - GHY .. y0 + RTY - 1 + GHY
+    static constexpr int GY = RTY + 2 * GHY, AY = GY + 2;  // snapshot rows: one more on each side
 };
 
 template <int S, bool SUB>
@@ -292,17 +295,21 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                                                  const uint16_t* __restrict__ info_table, uint8_t* __restrict__ out)
 {
     using Win = RasterWin<SUB>;
-    constexpr int GX = Win::GX, GY = Win::GY, AX = Win::AX, AY = Win::AY, GHX = Win::GHX, GHY = Win::GHY;
+    constexpr int GY = Win::GY, AY = Win::AY, GHY = Win::GHY;
     constexpr int MW = mask_w(S), KMIN = mask_kmin(S);
-    __shared__ uint8_t s_aux[AX * AY];
-    __shared__ uint8_t s_g[GX * GY];
+    __shared__ uint32_t s_aux32[AY * RWW];
+    __shared__ uint32_t s_p32[(GY + 1) * RWW];   // decisions of crossed blocks, by origin (snapshot rows 0..GY)
+    __shared__ uint16_t s_blk[(GY + 1) * 40];   // crossed blocks: origin byte index in the window
+    __shared__ int s_nblk;
+    __shared__ uint32_t s_g32[GY * RWW];
     __shared__ uint16_t s_key[RCX * RCY];
-    __shared__ uint16_t s_info[RCX * RCY];
+    __shared__ uint8_t s_info[RCX * RCY];  // coverage summary bits 0..4 (simple_cell)
     __shared__ uint32_t s_col[RCX * RCY];
     __shared__ uint32_t s_mask[SUB ? RCX * RCY * MW : 1];  // coverage rows of the smoothed candidates
     __shared__ int8_t s_hull[SUB ? RHX * RHY * 17 : 1];
     __shared__ uint16_t s_list[RTX * RTY];  // own cells that need the candidate search
     __shared__ int s_nlist;
+    const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
 
     const int t = threadIdx.x;
     int gi, tile;
@@ -310,56 +317,128 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     const DevItem it = items[group[gi]];
     const int tiles_x = (it.W + RTX - 1) / RTX;
     const int x0 = (tile % tiles_x) * RTX, y0 = (tile / tiles_x) * RTY;
-    ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags };
     const uint8_t* aux = gaux + it.g_off;
-    const int ax0 = x0 - GHX - 1, ay0 = y0 - GHY - 1;  // origin of the snapshot window
+    const int ay0 = y0 - GHY - 1;  // first snapshot row
 
-    if (t == 0) s_nlist = 0;
-    // the snapshot bytes and the candidates' colours are independent loads:
-    // issue them together
-    for (int k = t; k < AX * AY; k += 256) {
-        int x = ax0 + k % AX, y = ay0 + k / AX;
-        bool in_frame = (unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H;
-        s_aux[k] = in_frame ? aux[(uint32_t)(y * it.W + x)] : 0;
+    if (t == 0) {
+        s_nlist = 0;
+        s_nblk = 0;
     }
-    for (int k = t; k < RCX * RCY; k += 256) {
-        int x = x0 - 1 + k % RCX, y = y0 - 1 + k / RCX;
-        if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) s_col[k] = cell_rgba(load_px(im, x, y), it.ch);
-    }
-    __syncthreads();
-    // heuristic crossing resolution (ambiguous_cross_Kernel) on the snapshot;
-    // the walks read the snapshot in LDS where they can, else in HBM
-    for (int k = t; k < GX * GY; k += 256) {
-        int lx = k % GX, ly = k / GX;
-        int c = (ly + 1) * AX + lx + 1;
-        uint32_t own = s_aux[c];
-        if (own & 0xA5u) {  // any diagonal bit (0, 2, 5, 7); off-frame bytes are 0
-            uint32_t nb[8];
-#pragma unroll
-            for (int e = 0; e < 8; e++) nb[e] = s_aux[c + edy(e) * AX + edx(e)];
-            auto get = [&](int x, int y) -> uint32_t {
-                int wx = x - ax0, wy = y - ay0;
-                if ((unsigned)wx < (unsigned)AX && (unsigned)wy < (unsigned)AY) return s_aux[wy * AX + wx];
-                return aux[(uint32_t)(y * it.W + x)];
-            };
-            own &= ~heuristic_clear_mask(get, ax0 + 1 + lx, ay0 + 1 + ly, own, nb);
+    if (t < (GY + 1) * RWW) s_p32[t] = 0;
+    // The snapshot bytes and the candidates' colours are independent loads:
+    // issue them together.  With W % 4 == 0 the graph rows are word aligned
+    // and a word is either wholly inside the frame or wholly outside.
+    if (it.graph_w4) {
+        if (t < AY * RWW) {
+            int r = t / RWW, w = t % RWW;
+            int x = x0 - 8 + 4 * w, y = ay0 + r;
+            uint32_t v = 0;
+            if (w >= 1 && w <= 10 && (unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
+                v = *reinterpret_cast<const uint32_t*>(aux + (uint32_t)(y * it.W + x));
+            s_aux32[t] = v;
         }
-        s_g[k] = (uint8_t)own;
-        // the resolved graph of the tile's own cells
-        int cx = lx - GHX, cy = ly - GHY;
-        if ((unsigned)cx < (unsigned)RTX && (unsigned)cy < (unsigned)RTY && x0 + cx < it.W && y0 + cy < it.H)
-            gfin[it.g_off + (uint32_t)((y0 + cy) * it.W + x0 + cx)] = (uint8_t)own;
+    } else {
+        uint8_t* s8 = reinterpret_cast<uint8_t*>(s_aux32);
+        for (int k = t; k < AY * RWB; k += 256) {
+            int r = k / RWB, c = k % RWB;
+            int x = x0 - 8 + c, y = ay0 + r;
+            bool ok = c >= 4 && c < 44 && (unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H;
+            s8[k] = ok ? aux[(uint32_t)(y * it.W + x)] : 0;
+        }
+    }
+    {
+        const uint8_t* base = in + it.in_off;
+        const uint32_t o0 = (it.flags & DPX_RGB) ? 2 : 0, o2 = 2 - o0;
+        const bool topdown = it.flags & DPX_TOPDOWN, four = it.ch == 4;
+        for (int k = t; k < RCX * RCY; k += 256) {
+            int x = x0 - 1 + k % RCX, y = y0 - 1 + k / RCX;
+            if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
+                int row = topdown ? it.H - 1 - y : y;
+                uint32_t off = __umul24(row, it.stride) + __umul24(x, it.ch);  // operands < 2^24
+                uint32_t a = four ? base[off + 3] : 255u;
+                s_col[k] = (uint32_t)base[off + o2] | (uint32_t)base[off + 1] << 8 | (uint32_t)base[off + o0] << 16 |
+                           a << 24;  // colorPos (kernel.cu:98-101)
+            }
+        }
     }
     __syncthreads();
-    // candidate cells: key and coverage summary
+    // Heuristic crossing resolution (ambiguous_cross_Kernel).  Only the
+    // pixels of crossed 2x2 blocks (both diagonals present) change, and in
+    // the symmetric snapshot the block with lower-left pixel p is crossed iff
+    // p has bit 2 and its right neighbour bit 0.  1: list the crossed blocks
+    // of the window, four origins of a row per thread; 2: decide each listed
+    // block once (processHeuristics2), the walks reading the snapshot in LDS
+    // where they can and in HBM beyond the window; 3: remove the losing
+    // diagonals, four positions per thread.
+    for (int k = t; k < (GY + 1) * 10; k += 256) {
+        const int r = k / 10, w = k % 10 + 1;  // word w of snapshot row r: origins of blocks on rows r, r + 1
+        const uint32_t* a = s_aux32 + r * RWW + w;
+        uint32_t D = a[0];
+        uint32_t cross = (D >> 2) & __builtin_amdgcn_alignbyte(a[1], D, 1) & 0x01010101u;
+        while (cross) {
+            int q = __builtin_ctz(cross) >> 3;
+            cross &= cross - 1;
+            s_blk[atomicAdd(&s_nblk, 1)] = (uint16_t)(r * RWB + 4 * w + q);
+        }
+    }
+    __syncthreads();
+    {
+        // (the window is read by words, so that the compiler keeps the LDS
+        // and HBM reads apart instead of merging them into one generic load)
+        auto get = [&](int gx, int gy) -> uint32_t {
+            int wx = gx - (x0 - 4), wy = gy - ay0;
+            bool win = (unsigned)wx < 40u && (unsigned)wy < (unsigned)AY;
+            int c = win ? wy * RWB + wx + 4 : 0;
+            uint32_t v = (s_aux32[c >> 2] >> (8 * (c & 3))) & 0xFFu;
+            if (!win) v = aux[(uint32_t)(gy * it.W + gx)];
+            return v;
+        };
+        const int nblk = s_nblk;
+        uint8_t* p8 = reinterpret_cast<uint8_t*>(s_p32);
+        for (int k = t; k < nblk; k += 256) {
+            int e = s_blk[k];
+            int r = e / RWB, c = e % RWB;
+            // 1: "/" loses (bits 2 and 5), 2: "\" loses (bits 0 and 7)
+            p8[e] = remove_slash(get, x0 - 8 + c, ay0 + r) ? 1 : 2;
+        }
+    }
+    __syncthreads();
+    for (int k = t; k < GY * 10; k += 256) {
+        const int r = k / 10, w = k % 10 + 1;  // word w of resolved row r (snapshot row r + 1)
+        const int y = ay0 + 1 + r, x = x0 - 8 + 4 * w;
+        const uint32_t* pa = s_p32 + (r + 1) * RWW + w;  // decisions of the blocks with origin on row y
+        const uint32_t* pb = pa - RWW;                   // ... on row y - 1
+        uint32_t A = pa[0], Al = __builtin_amdgcn_alignbyte(A, pa[-1], 3);
+        uint32_t B = pb[0], Bl = __builtin_amdgcn_alignbyte(B, pb[-1], 3);
+        uint32_t clear = ((A & 0x01010101u) << 2) |        // up-right block: "/"
+                         ((Al >> 1) & 0x01010101u) |        // up-left block: "\"
+                         ((Bl & 0x01010101u) << 5) |        // down-left block: "/"
+                         (((B >> 1) & 0x01010101u) << 7);  // down-right block: "\"
+        uint32_t res = s_aux32[(r + 1) * RWW + w] & ~clear;
+        s_g32[r * RWW + w] = res;
+        // the resolved graph of the tile's own cells
+        if (y >= y0 && y < y0 + RTY && w >= 2 && w <= 9 && y < it.H && x < it.W) {
+            uint8_t* dst = gfin + it.g_off + (uint32_t)(y * it.W + x);
+            if (it.graph_w4) {
+                *reinterpret_cast<uint32_t*>(dst) = res;
+            } else {
+                int n = min(4, it.W - x);
+                for (int q = 0; q < n; q++) dst[q] = (uint8_t)(res >> (8 * q));
+            }
+        }
+    }
+    __syncthreads();
+    // candidate cells: key and coverage summary.  Candidate (lx, ly) is the
+    // cell at (x0 - 1 + lx, y0 - 1 + ly): resolved row ly - 1 + GHY, byte lx + 7.
     for (int k = t; k < RCX * RCY; k += 256) {
         int lx = k % RCX, ly = k / RCX;
         int x = x0 - 1 + lx, y = y0 - 1 + ly;
-        uint16_t key = kNoCell, info = 0;
+        uint16_t key = kNoCell;
+        uint8_t info = 0;
         if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
-            int c = (ly - 1 + GHY) * GX + lx - 1 + GHX;
+            int c = (ly - 1 + GHY) * RWB + lx + 7;
             key = (uint16_t)cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
-            info = info_table[key];
+            info = (uint8_t)(info_table[key] & 0x1Fu);
             if constexpr (SUB) {
                 const uint32_t* row = mask_table + (size_t)key * MW;
 #pragma unroll
@@ -370,9 +449,10 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         s_info[k] = info;
     }
     if constexpr (SUB) {
-        for (int k = t; k < RHX * RHY; k += 256) {
+        ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags };
+        for (int k = t; k < RHX * RHY; k += 256) {  // hull (lx, ly): cell (x0 - 2 + lx, y0 - 2 + ly)
             int lx = k % RHX, ly = k / RHX;
-            int c = (ly - 2 + GHY) * GX + lx - 2 + GHX;
+            int c = (ly - 2 + GHY) * RWB + lx + 6;
             uint32_t key = cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
             const int8_t* e = hull_table + (size_t)key * kHullBytes;
 #pragma unroll
@@ -385,7 +465,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             if (s_key[k] == kNoCell) continue;
             int lx = k % RCX, ly = k / RCX;
             int x = x0 - 1 + lx, y = y0 - 1 + ly;
-            uint32_t node = s_g[(ly - 1 + GHY) * GX + lx - 1 + GHX];
+            uint32_t node = s_g[(ly - 1 + GHY) * RWB + lx + 7];
             if (node == 90) continue;  // internal node keeps its hull (kernel.cu:231)
             int hk = (ly + 1) * RHX + lx + 1;
             int n = s_hull[hk * 17];
@@ -408,7 +488,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 rows[r] = coverage_row(ox, oy, tri, nt, S, r);
                 s_mask[k * MW + r] = rows[r];
             }
-            s_info[k] = (uint16_t)coverage_info(rows, S);
+            s_info[k] = (uint8_t)(coverage_info(rows, S) & 0x1Fu);
         }
     }
     __syncthreads();
@@ -444,41 +524,30 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         for (int q = 0; q < S; q++) __builtin_nontemporal_store(col[q], reinterpret_cast<uint32_t*>(dst + 4 * q));
     };
 
-    // phase A: each own cell whose S*S pixels are all its own colour writes
-    // them directly; the others go to a list
-    for (int oc = t; oc < RTX * RTY; oc += 256) {
-        int cx = oc % RTX, cy = oc / RTX;
-        int gx = x0 + cx, gy = y0 + cy;
-        if (gx < it.W && gy < it.H) {
-            int k = (cy + 1) * RCX + cx + 1;
-            if (simple_cell(s_info[k], s_info[k + RCX - 1], s_info[k + RCX], s_info[k + RCX + 1], s_info[k + 1])) {
-                uint32_t col[S];
-#pragma unroll
-                for (int q = 0; q < S; q++) col[q] = s_col[k];
-#pragma unroll
-                for (int ky = 0; ky < S; ky++) store_row(gx, gy * S + ky, col);
-            } else {
-                s_list[atomicAdd(&s_nlist, 1)] = (uint16_t)oc;
-            }
-        }
-    }
+    // Own cells, one per thread: a cell whose S*S pixels all show its own
+    // colour is written directly, the others are listed for the candidate
+    // search.  The list is complete before any pixel is written, so the
+    // searches' table reads are in flight while the direct writes issue.
+    const int cx = t % RTX, cy = t / RTX;
+    const int kown = (cy + 1) * RCX + cx + 1;
+    const bool mine = x0 + cx < it.W && y0 + cy < it.H;
+    const bool simple = mine && simple_cell(s_info[kown], s_info[kown + RCX - 1], s_info[kown + RCX],
+                                            s_info[kown + RCX + 1], s_info[kown + 1]);
+    if (mine && !simple) s_list[atomicAdd(&s_nlist, 1)] = (uint16_t)t;
     __syncthreads();
 
-    // phase B: one unit = one output row (S pixels) of a listed cell, resolved
+    // One unit = one output row (S pixels) of a listed cell, resolved
     // against the 3x3 candidates in draw order (highest cell index first).
-    // The candidates' coverage rows are all fetched before they are combined.
     constexpr uint32_t FULL = (S >= 32) ? 0xFFFFFFFFu : ((1u << S) - 1u);
     const int nunits = s_nlist * S;
-    for (int u = t; u < nunits; u += 256) {
+    auto fetch_rows = [&](int u, uint32_t* rows) {
         int own = s_list[u / S], ky = u % S;
-        int cx = own % RTX, cy = own / RTX;
-        int gx = x0 + cx, gy = y0 + cy;
-        uint32_t rows[9];
+        int ux = own % RTX, uy = own / RTX;
 #pragma unroll
         for (int dy = 1; dy >= -1; dy--) {
 #pragma unroll
             for (int dx = 1; dx >= -1; dx--) {
-                int k = (cy + 1 + dy) * RCX + (cx + 1 + dx);
+                int k = (uy + 1 + dy) * RCX + (ux + 1 + dx);
                 int r = ky - dy * S - KMIN;
                 uint32_t row = 0;
                 if (r >= 0 && r < MW && s_key[k] != kNoCell) {
@@ -488,6 +557,10 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 rows[(1 - dy) * 3 + (1 - dx)] = row;
             }
         }
+    };
+    auto resolve_unit = [&](int u, const uint32_t* rows) {
+        int own = s_list[u / S], ky = u % S;
+        int ux = own % RTX, uy = own / RTX;
         uint32_t col[S];
 #pragma unroll
         for (int q = 0; q < S; q++) col[q] = 0xFF000000u;
@@ -501,7 +574,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 uint32_t bits = (sh >= 0 ? (row >> sh) : (row << -sh)) & FULL;
                 uint32_t nb = bits & ~assigned;
                 if (nb) {
-                    uint32_t c = s_col[(cy + 1 + dy) * RCX + (cx + 1 + dx)];
+                    uint32_t c = s_col[(uy + 1 + dy) * RCX + (ux + 1 + dx)];
 #pragma unroll
                     for (int q = 0; q < S; q++)
                         if ((nb >> q) & 1u) col[q] = c;
@@ -509,7 +582,24 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 }
             }
         }
-        store_row(gx, gy * S + ky, col);
+        store_row(x0 + ux, (y0 + uy) * S + ky, col);
+    };
+
+    uint32_t rows0[9];
+    if (t < nunits) fetch_rows(t, rows0);
+    if (simple) {
+        uint32_t col[S];
+        uint32_t c = s_col[kown];
+#pragma unroll
+        for (int q = 0; q < S; q++) col[q] = c;
+#pragma unroll
+        for (int ky = 0; ky < S; ky++) store_row(x0 + cx, (y0 + cy) * S + ky, col);
+    }
+    if (t < nunits) resolve_unit(t, rows0);
+    for (int u = t + 256; u < nunits; u += 256) {
+        uint32_t rows[9];
+        fetch_rows(u, rows);
+        resolve_unit(u, rows);
     }
 }
 

@@ -144,10 +144,16 @@ __device__ inline bool simple_cell(uint32_t own, uint32_t upl, uint32_t up, uint
 }
 
 // ---------------------------------------------------------------------------
-// work distribution: block b -> (item, tile).  When every item of a launch
-// has the same number of tiles (`uniform`), it is a division; otherwise a
-// binary search over the prefix of tile counts.
+// work distribution: block -> (item, tile column, tile row).  When every item
+// of a launch has the same tile grid, the launch grid is (tile columns, tile
+// rows, items) and the block index says it all; otherwise the grid is 1-D
+// over all tiles and the item is found by binary search over the prefix of
+// tile counts.
 // ---------------------------------------------------------------------------
+struct TileAt {
+    int item, tx, ty;
+};
+
 __device__ inline int find_item(const uint32_t* __restrict__ prefix, int n, uint32_t b)
 {
     int lo = 0, hi = n;  // prefix[lo] <= b < prefix[hi]
@@ -158,16 +164,15 @@ __device__ inline int find_item(const uint32_t* __restrict__ prefix, int n, uint
     return lo;
 }
 
-__device__ inline void block_item(const uint32_t* __restrict__ prefix, int n, uint32_t uniform, uint32_t b,
-                                  int* item, int* tile)
+template <int TX>
+__device__ inline TileAt locate_tile(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix, int n,
+                                     int grid3d)
 {
-    if (uniform) {
-        *item = (int)(b / uniform);
-        *tile = (int)(b % uniform);
-    } else {
-        *item = find_item(prefix, n, b);
-        *tile = (int)(b - prefix[*item]);
-    }
+    if (grid3d) return TileAt{ (int)blockIdx.z, (int)blockIdx.x, (int)blockIdx.y };
+    int item = find_item(prefix, n, blockIdx.x);
+    int tile = (int)(blockIdx.x - prefix[item]);
+    int tiles_x = (items[item].W + TX - 1) / TX;
+    return TileAt{ item, tile % tiles_x, tile / tiles_x };
 }
 
 // ---------------------------------------------------------------------------
@@ -192,18 +197,16 @@ constexpr int GHX = GTX + 1, GHY = GTY + 2;  // positions of pass 2
 constexpr int GHP = GTX + 4;                 // byte pitch of pass 2's rows: column c is x = x0 - 4 + c
 
 __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix,
-                                                int n_items, uint32_t uniform, const uint8_t* __restrict__ in,
+                                                int n_items, int grid3d, const uint8_t* __restrict__ in,
                                                 uint8_t* __restrict__ gaux)
 {
     __shared__ uint32_t s_lane[GLX * GLY];
     __shared__ uint32_t s_h32[GHY * GHP / 4];
     uint8_t* s_h = reinterpret_cast<uint8_t*>(s_h32);
     const int t = threadIdx.x;
-    int ii, tile;
-    block_item(prefix, n_items, uniform, blockIdx.x, &ii, &tile);
-    const DevItem it = items[ii];
-    const int tiles_x = (it.W + GTX - 1) / GTX;
-    const int x0 = (tile % tiles_x) * GTX, y0 = (tile / tiles_x) * GTY;
+    const TileAt at = locate_tile<GTX>(items, prefix, n_items, grid3d);
+    const DevItem it = items[at.item];
+    const int x0 = at.tx * GTX, y0 = at.ty * GTY;
 
     // pass 1.  Byte offsets within an item fit 31 bits (checked on the host).
     const uint8_t* base = in + it.in_off;
@@ -286,9 +289,10 @@ struct RasterWin {
     static constexpr int GY = RTY + 2 * GHY, AY = GY + 2;  // snapshot rows: one more on each side
 };
 
+// `items` are the items of one (scale, subdivision) group, in launch order.
 template <int S, bool SUB>
-__global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const int32_t* __restrict__ group,
-                                                 const uint32_t* __restrict__ prefix, int n_group, uint32_t uniform,
+__global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix,
+                                                 int n_items, int grid3d,
                                                  const uint8_t* __restrict__ in, const uint8_t* __restrict__ gaux,
                                                  uint8_t* __restrict__ gfin, const int8_t* __restrict__ hull_table,
                                                  const uint32_t* __restrict__ mask_table,
@@ -312,11 +316,9 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
 
     const int t = threadIdx.x;
-    int gi, tile;
-    block_item(prefix, n_group, uniform, blockIdx.x, &gi, &tile);
-    const DevItem it = items[group[gi]];
-    const int tiles_x = (it.W + RTX - 1) / RTX;
-    const int x0 = (tile % tiles_x) * RTX, y0 = (tile / tiles_x) * RTY;
+    const TileAt at = locate_tile<RTX>(items, prefix, n_items, grid3d);
+    const DevItem it = items[at.item];
+    const int x0 = at.tx * RTX, y0 = at.ty * RTY;
     const uint8_t* aux = gaux + it.g_off;
     const int ay0 = y0 - GHY - 1;  // first snapshot row
 
@@ -603,8 +605,8 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     }
 }
 
-using RasterFn = void (*)(const DevItem*, const int32_t*, const uint32_t*, int, uint32_t, const uint8_t*,
-                          const uint8_t*, uint8_t*, const int8_t*, const uint32_t*, const uint16_t*, uint8_t*);
+using RasterFn = void (*)(const DevItem*, const uint32_t*, int, int, const uint8_t*, const uint8_t*, uint8_t*,
+                          const int8_t*, const uint32_t*, const uint16_t*, uint8_t*);
 
 template <int S>
 RasterFn raster_fn(bool sub)
@@ -682,20 +684,25 @@ int ensure_mask(dpx_context* ctx, int S)
 struct Group {
     int scale;
     bool sub;
-    std::vector<int32_t> idx;
+    std::vector<DevItem> dev;  // the group's items, in launch order
     std::vector<uint32_t> prefix;
 };
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 // Tiles per item when all items of a prefix have the same count, else 0.
-uint32_t uniform_tiles(const std::vector<uint32_t>& prefix)
+// Launch grid of a kernel with TX x TY tiles over `dev`: (tile columns, tile
+// rows, items) when all items have the same tile grid, else 1-D over the
+// prefix of tile counts.  *grid3d tells the kernel which.
+template <int TX, int TY>
+dim3 tile_grid(const std::vector<DevItem>& dev, const std::vector<uint32_t>& prefix, int* grid3d)
 {
-    if (prefix.size() < 2) return 0;
-    uint32_t d = prefix[1] - prefix[0];
-    for (size_t k = 2; k < prefix.size(); k++)
-        if (prefix[k] - prefix[k - 1] != d) return 0;
-    return d;
+    const unsigned tx = (unsigned)(dev[0].W + TX - 1) / TX, ty = (unsigned)(dev[0].H + TY - 1) / TY;
+    bool same = dev.size() <= 65535 && ty <= 65535;
+    for (const DevItem& d : dev)
+        same = same && (unsigned)(d.W + TX - 1) / TX == tx && (unsigned)(d.H + TY - 1) / TY == ty;
+    *grid3d = same ? 1 : 0;
+    return same ? dim3(tx, ty, (unsigned)dev.size()) : dim3(prefix.back(), 1, 1);
 }
 
 }  // namespace
@@ -799,7 +806,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         grp.scale = a.scale;
         grp.sub = d.subdivide;
         if (grp.prefix.empty()) grp.prefix.push_back(0);
-        grp.idx.push_back(k);
+        grp.dev.push_back(d);
         uint32_t rt = (uint32_t)(((a.width + RTX - 1) / RTX) * ((a.height + RTY - 1) / RTY));
         grp.prefix.push_back(grp.prefix.back() + rt);
     }
@@ -819,7 +826,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         ctx->g_cap = cap;
     }
 
-    // descriptor blob: items | gprefix | per group (idx, prefix)
+    // descriptor blob: items | gprefix | per group (items, prefix)
     std::vector<uint8_t> blob;
     auto put = [&](const void* p, size_t n) {
         size_t off = align_up(blob.size(), 64);
@@ -831,7 +838,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
     size_t off_gprefix = put(gprefix.data(), gprefix.size() * sizeof(uint32_t));
     std::vector<std::pair<size_t, size_t>> goffs;
     for (auto& kv : groups) {
-        size_t a = put(kv.second.idx.data(), kv.second.idx.size() * sizeof(int32_t));
+        size_t a = put(kv.second.dev.data(), kv.second.dev.size() * sizeof(DevItem));
         size_t b = put(kv.second.prefix.data(), kv.second.prefix.size() * sizeof(uint32_t));
         goffs.push_back({ a, b });
     }
@@ -872,20 +879,22 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         ctx->ev_used += 3;
         DPX_HIP(hipEventRecord(ev[0], stream));
     }
-    hipLaunchKernelGGL(kf_graph, dim3(gprefix[n_items]), dim3(256), 0, stream, d_items, d_gprefix, n_items,
-                       uniform_tiles(gprefix), d_in, ctx->d_gaux);
+    int grid3d = 0;
+    dim3 ggrid = tile_grid<GTX, GTY>(dev, gprefix, &grid3d);
+    hipLaunchKernelGGL(kf_graph, ggrid, dim3(256), 0, stream, d_items, d_gprefix, n_items, grid3d, d_in,
+                       ctx->d_gaux);
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[1], stream));
     int gk = 0;
     for (auto& kv : groups) {
         Group& grp = kv.second;
         RasterFn fn = pick_raster(grp.scale, grp.sub);
         DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", grp.scale);
-        const int32_t* d_idx = reinterpret_cast<const int32_t*>(ctx->d_desc + goffs[gk].first);
+        const DevItem* d_gitems = reinterpret_cast<const DevItem*>(ctx->d_desc + goffs[gk].first);
         const uint32_t* d_rp = reinterpret_cast<const uint32_t*>(ctx->d_desc + goffs[gk].second);
-        hipLaunchKernelGGL(fn, dim3(grp.prefix.back()), dim3(256), 0, stream, d_items, d_idx, d_rp,
-                           (int)grp.idx.size(), uniform_tiles(grp.prefix), d_in, ctx->d_gaux, ctx->d_gfin,
-                           ctx->d_hull, ctx->d_mask[grp.scale],
-                           ctx->d_info[grp.scale], d_out);
+        dim3 rgrid = tile_grid<RTX, RTY>(grp.dev, grp.prefix, &grid3d);
+        hipLaunchKernelGGL(fn, rgrid, dim3(256), 0, stream, d_gitems, d_rp, (int)grp.dev.size(), grid3d, d_in,
+                           ctx->d_gaux, ctx->d_gfin, ctx->d_hull, ctx->d_mask[grp.scale], ctx->d_info[grp.scale],
+                           d_out);
         gk++;
     }
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[2], stream));

@@ -134,6 +134,16 @@ int dpx_stage_rasterize(const uint8_t* d_img, int width, int height, int stride,
  * index of k's component */
 int dpx_stage_components(const uint8_t* d_graph, int width, int height, int32_t* d_labels,
                          void* stream);
+/* border chains of a resolved graph: extractBorderPoints cc_functions.cu:348-503
+ * (its CClist / CCsizes outputs).  Chain k is d_nodes[sum of d_sizes[0..k-1] ..]
+ * of d_sizes[k] node indices j*width+i, chains in the order the row-major scan
+ * starts them.  d_scratch: width*height bytes.  d_counts[3] receives the
+ * number of chains, of nodes and a status (0; DPX_ENOMEM when max_nodes or
+ * max_chains is too small; DPX_EINVAL when the graph is not a resolved,
+ * symmetric similarity graph), readable once the stream reaches this point. */
+int dpx_stage_border_chains(const uint8_t* d_graph, int width, int height, uint8_t* d_scratch,
+                            int32_t* d_nodes, int max_nodes, int32_t* d_sizes, int max_chains,
+                            int32_t* d_counts, void* stream);
 
 /* ------------------------------------------------------------------------ */
 /* Batched path: many frames / sprites of uneven sizes through one fused     */

@@ -201,6 +201,24 @@ def components(g: np.ndarray) -> np.ndarray:
     return lab
 
 
+def border_chains(g: np.ndarray) -> list:
+    """Border chains of a resolved graph in scan order, as lists of node
+    indices j * W + i (buffer row order)."""
+    g = np.ascontiguousarray(g, dtype=np.uint8)
+    H, W = g.shape
+    cap = 8 * W * H + 8
+    nodes = np.zeros(cap, dtype=np.int32)
+    sizes = np.zeros(W * H + 1, dtype=np.int32)
+    nc, nn = c_int(0), c_int(0)
+    _ok(lib().orc_border_chains(_p(g), W, H, _p(nodes), _p(sizes), cap, sizes.size, ctypes.byref(nc),
+                                ctypes.byref(nn)), "border_chains")
+    out, k = [], 0
+    for s in sizes[:nc.value]:
+        out.append(nodes[k:k + s].tolist())
+        k += s
+    return out
+
+
 # --------------------------------------------------------------------------
 # reference harness (the reference's own device functions, run on a GPU)
 # --------------------------------------------------------------------------

@@ -988,3 +988,93 @@ int orc_components(const uint8_t* g, int W, int H, int32_t* labels)
     free(queue);
     return ORC_OK;
 }
+
+/* ------------------------------------------------------------------------- */
+/* A18  border chains (extractBorderPoints cc_functions.cu:348-503)           */
+/*   A row-major scan over the nodes; a node that is not internal (90) and    */
+/*   not yet listed starts a walk along the border of its face: leave by its  */
+/*   first link in clockwise order from up-left (getFirstLink 101-113 with    */
+/*   getRealLinkIndex 64-91), and at every node arrived at by edge a, leave   */
+/*   by the next link clockwise after a (nextNodeClockwise 285-309; a node    */
+/*   with one link sends the walk back).  The walk is a chain when it comes   */
+/*   back to its start through the link just counter-clockwise of the first   */
+/*   one (nextEdgeCounterClockwise 250-275, the loop test at 428-429); it is  */
+/*   abandoned when it steps onto an internal, listed or discarded node, and  */
+/*   then the node it stepped from is discarded (449-457 store the last node  */
+/*   of the walk numCCNodes times).  The nodes of a chain are listed.         */
+/*   Where the shipped code and its recorded output disagree the output wins: */
+/*   cc_functions.cu:417 lists a walk's start before walking, yet the chains  */
+/*   in alex_png.txt:1-21 (the alex graph, kernel.cu:296) are reproduced only */
+/*   if an abandoned walk leaves its start unlisted (with the start listed,   */
+/*   the walks from 34 and 82 block the chains that start at 35 and 83).      */
+/*   Canonical for a node with no links, where the reference indexes a        */
+/*   neighbour with edge -1 (undefined): it starts no walk.                   */
+/*   Pinned by: alex_png.txt:1-21, all 21 chains and their order.             */
+/* ------------------------------------------------------------------------- */
+
+/* clockwise position (from up-left) -> edge bit, and edge bit -> position */
+static const int CW_BIT[8] = { 0, 1, 2, 4, 7, 6, 5, 3 };
+static const int CW_POS[8] = { 0, 1, 2, 7, 3, 6, 5, 4 };
+
+/* The link of node value v next to edge a, turning clockwise (dir +1) or
+   counter-clockwise (dir -1); a itself when it is the only link. */
+static int turn_link(uint32_t v, int a, int dir)
+{
+    int p = CW_POS[a];
+    for (int k = 1; k < 8; k++) {
+        int b = CW_BIT[(p + dir * k + 8) % 8];
+        if (BIT(v, b)) return b;
+    }
+    return a;
+}
+
+int orc_border_chains(const uint8_t* g, int W, int H, int32_t* nodes, int32_t* sizes, int max_nodes,
+                      int max_chains, int* n_chains, int* n_nodes)
+{
+    if (!g || W <= 0 || H <= 0 || !n_chains || !n_nodes) return ORC_EINVAL;
+    if (!graph_valid(g, W, H)) return ORC_EGRAPH;
+    const long N = (long)W * H;
+    uint8_t* mark = (uint8_t*)calloc((size_t)N, 1); /* 1 listed, 2 discarded */
+    int32_t* walk = (int32_t*)malloc((size_t)(8 * N + 1) * sizeof(int32_t));
+    int rc = ORC_OK;
+    int nc = 0, nn = 0;
+    if (!mark || !walk) { rc = ORC_EINVAL; goto done; }
+    for (long s = 0; s < N; s++) {
+        uint32_t v = g[s];
+        if (v == 90 || v == 0 || mark[s] == 1) continue;
+        int e = -1;
+        for (int p = 0; p < 8 && e < 0; p++)
+            if (BIT(v, CW_BIT[p])) e = CW_BIT[p];
+        const int arrive = turn_link(v, e, -1);
+        long len = 0, cur = s;
+        int ok = 1;
+        walk[len++] = (int32_t)s;
+        for (;;) {
+            long nxt = cur + (long)EDY[e] * W + EDX[e];
+            if (nxt == s && 7 - e == arrive) break;  /* closed */
+            cur = nxt;
+            e = turn_link(g[cur], 7 - e, +1);
+            if (g[cur] == 90 || mark[cur]) {
+                mark[walk[len - 1]] = 2;
+                ok = 0;
+                break;
+            }
+            if (len > 8 * N) { rc = ORC_EGRAPH; goto done; } /* not a border: the graph is not symmetric */
+            walk[len++] = (int32_t)cur;
+        }
+        if (!ok) continue;
+        if (nc >= max_chains || nn + len > max_nodes) { rc = ORC_ESPACE; goto done; }
+        for (long k = 0; k < len; k++) {
+            nodes[nn + k] = walk[k];
+            mark[walk[k]] = 1;
+        }
+        sizes[nc++] = (int32_t)len;
+        nn += (int)len;
+    }
+done:
+    *n_chains = nc;
+    *n_nodes = nn;
+    free(mark);
+    free(walk);
+    return rc;
+}

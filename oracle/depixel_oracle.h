@@ -37,7 +37,8 @@ typedef struct { float x, y; } orc_pt;
 /* status codes */
 #define ORC_OK            0
 #define ORC_EINVAL       -1
-#define ORC_EGRAPH       -2   /* graph has an edge leaving the frame */
+#define ORC_EGRAPH       -2   /* graph has an edge leaving the frame, or a border walk that never closes */
+#define ORC_ESPACE       -3   /* an output array is too small */
 
 /* ---- per-colour helpers (A2) ---- */
 uint32_t orc_rgb_to_yuv(uint32_t packed_c0c1c2);
@@ -74,8 +75,11 @@ int orc_depixelize(const uint8_t* img, int W, int H, int stride, int channels, i
 
 /* ---- connected components and boundary chains (A18) ---- */
 int orc_components(const uint8_t* graph, int W, int H, int32_t* labels);
+/* Border chains in scan order: chain k is chain_nodes[sum of earlier sizes ..]
+   of chain_sizes[k] node indices (j * W + i in buffer row order).  *n_chains
+   and *n_nodes are the totals; ORC_ESPACE if they exceed max_chains/max_nodes. */
 int orc_border_chains(const uint8_t* graph, int W, int H, int32_t* chain_nodes,
-                      int32_t* chain_sizes, int max_nodes, int max_chains, int* n_chains);
+                      int32_t* chain_sizes, int max_nodes, int max_chains, int* n_chains, int* n_nodes);
 
 #ifdef __cplusplus
 }

@@ -111,4 +111,38 @@ __host__ __device__ inline uint32_t heuristic_clear_mask(G get, int i, int j, ui
     return clear;
 }
 
+// Border walks (cc_functions.cu): links in clockwise order from up-left are
+// bits 0, 1, 2, 4, 7, 6, 5, 3 (getRealLinkIndex 64-91, getClockLinkIndex
+// 154-184).  cw_bit(p) is the bit at clockwise position p, cw_pos its inverse.
+__host__ __device__ inline int cw_bit(int p)
+{
+    return (int)((0x35674210u >> (4 * p)) & 0xFu);
+}
+
+__host__ __device__ inline int cw_pos(int b)
+{
+    return (int)((0x45637210u >> (4 * b)) & 0xFu);
+}
+
+// The first link of node v clockwise from up-left (getFirstLink 101-113), -1 if none.
+__host__ __device__ inline int cw_first_link(uint32_t v)
+{
+    for (int p = 0; p < 8; p++)
+        if (bit(v, cw_bit(p))) return cw_bit(p);
+    return -1;
+}
+
+// The link of node v next to link a turning clockwise (dir = +1,
+// nextNodeClockwise 285-309) or counter-clockwise (dir = -1,
+// nextEdgeCounterClockwise 250-275); a itself when it is the only link.
+__host__ __device__ inline int cw_turn(uint32_t v, int a, int dir)
+{
+    int p = cw_pos(a);
+    for (int k = 1; k < 8; k++) {
+        int b = cw_bit((p + dir * k + 8) & 7);
+        if (bit(v, b)) return b;
+    }
+    return a;
+}
+
 }  // namespace dpx

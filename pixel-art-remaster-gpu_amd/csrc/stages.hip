@@ -303,6 +303,91 @@ __global__ void k_cc_final(int32_t* parent, int32_t* labels, int N)
     if (k < N) labels[k] = uf_find(parent, k);
 }
 
+// Border chains (extractBorderPoints cc_functions.cu:348-503) of one frame,
+// one 64-lane block.  Which walks become chains depends on every earlier
+// walk (a walk is abandoned on a node an earlier chain listed or an earlier
+// abandoned walk discarded), so the scan is sequential by definition: the
+// lanes clear the marks and find start candidates 64 nodes at a time, and
+// lane 0 walks.  A walk starts at a node that is neither internal (90) nor
+// without links nor listed, leaves by its first link clockwise from
+// up-left, and at each node turns clockwise from the link it came in by.
+// It is a chain when it comes back to its start through the link just
+// counter-clockwise of the first one; it is abandoned on an internal,
+// listed or discarded node, discarding the node it stepped from.  Chain
+// nodes are listed; an abandoned walk leaves its start unlisted (the
+// behaviour that reproduces the chains recorded in alex_png.txt:1-21).
+// mark: 0 free, 1 listed, 2 discarded.  counts: chains, nodes, status
+// (0, or DPX_ENOMEM when the output arrays are too small, or DPX_EINVAL when
+// a walk leaves the frame or does not close: the graph is not a resolved,
+// symmetric similarity graph).
+__global__ __launch_bounds__(64) void k_border_chains(const uint8_t* __restrict__ graph, int W, int H,
+                                                      uint8_t* mark, int32_t* nodes, int32_t* sizes, int max_nodes,
+                                                      int max_chains, int32_t* counts)
+{
+    __shared__ int s_status;
+    const int N = W * H, lane = threadIdx.x;
+    for (int k = lane; k < N; k += 64) mark[k] = 0;
+    if (lane == 0) s_status = 0;
+    __syncthreads();
+    int nc = 0, nn = 0;  // lane 0's totals
+    for (int base = 0; base < N; base += 64) {
+        const int me = base + lane;
+        const uint32_t mv = me < N ? graph[me] : 90u;
+        uint64_t cand = __ballot(mv != 90u && mv != 0u);
+        if (lane == 0) {
+            while (cand && s_status == 0) {
+                const int s = base + __builtin_ctzll(cand);
+                cand &= cand - 1;
+                if (mark[s] == 1) continue;
+                const uint32_t v = graph[s];
+                int e = cw_first_link(v);
+                const int arrive = cw_turn(v, e, -1);
+                int ci = s % W, cj = s / W, cur = s, last = s;
+                long len = 1;
+                bool kept = true;
+                if (nn < max_nodes) nodes[nn] = s;
+                for (;;) {
+                    const int ni = ci + edx(e), nj = cj + edy(e);
+                    if (!in_frame(ni, nj, W, H) || len > 8L * N) {
+                        s_status = DPX_EINVAL;
+                        break;
+                    }
+                    const int nxt = nj * W + ni;
+                    if (nxt == s && 7 - e == arrive) break;  // back at the start: a chain
+                    const uint32_t nv = graph[nxt];
+                    if (nv == 90u || mark[nxt] != 0) {
+                        mark[last] = 2;
+                        kept = false;
+                        break;
+                    }
+                    e = cw_turn(nv, 7 - e, +1);
+                    ci = ni;
+                    cj = nj;
+                    cur = nxt;
+                    if (nn + len < max_nodes) nodes[nn + len] = cur;
+                    len++;
+                    last = cur;
+                }
+                if (s_status || !kept) continue;
+                if (nc >= max_chains || nn + len > max_nodes) {
+                    s_status = DPX_ENOMEM;
+                    continue;
+                }
+                for (long k = 0; k < len; k++) mark[nodes[nn + k]] = 1;
+                sizes[nc++] = (int32_t)len;
+                nn += (int)len;
+            }
+        }
+        __syncthreads();
+        if (s_status) break;
+    }
+    if (lane == 0) {
+        counts[0] = nc;
+        counts[1] = nn;
+        counts[2] = s_status;
+    }
+}
+
 int check_img(const void* p, int W, int H, int stride, int ch)
 {
     DPX_CHECK(p != nullptr, DPX_EINVAL, "null image pointer");
@@ -419,6 +504,20 @@ extern "C" int dpx_stage_components(const uint8_t* d_graph, int W, int H, int32_
     hipLaunchKernelGGL(k_cc_init, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, d_labels, N);
     hipLaunchKernelGGL(k_cc_union, grid2(W, H), dim3(kB, kB), 0, s, d_graph, W, H, d_labels);
     hipLaunchKernelGGL(k_cc_final, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, d_labels, d_labels, N);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_stage_border_chains(const uint8_t* d_graph, int W, int H, uint8_t* d_scratch, int32_t* d_nodes,
+                                       int max_nodes, int32_t* d_sizes, int max_chains, int32_t* d_counts,
+                                       void* stream)
+{
+    DPX_CHECK(d_graph && d_scratch && d_nodes && d_sizes && d_counts && W > 0 && H > 0, DPX_EINVAL,
+              "bad arguments");
+    DPX_CHECK((int64_t)W * H <= (1 << 28), DPX_EINVAL, "frame too large (%dx%d)", W, H);
+    DPX_CHECK(max_nodes >= 0 && max_chains >= 0, DPX_EINVAL, "negative capacity");
+    hipLaunchKernelGGL(k_border_chains, dim3(1), dim3(64), 0, as_stream(stream), d_graph, W, H, d_scratch, d_nodes,
+                       d_sizes, max_nodes, max_chains, d_counts);
     DPX_HIP(hipGetLastError());
     return DPX_OK;
 }

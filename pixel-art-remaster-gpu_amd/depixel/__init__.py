@@ -81,6 +81,8 @@ _SIGS = {
     "dpx_stage_rasterize": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_uint32, c_void_p, c_void_p, c_int,
                                     c_void_p, c_int, c_void_p]),
     "dpx_stage_components": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
+    "dpx_stage_border_chains": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int,
+                                        c_void_p, c_void_p]),
     "dpx_context_create": (c_int, [c_int, POINTER(c_void_p)]),
     "dpx_context_destroy": (None, [c_void_p]),
     "dpx_batch_run": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(Item), c_int, c_void_p]),

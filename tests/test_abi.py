@@ -21,7 +21,8 @@ def test_header_declares_the_drop_in_and_the_stages():
     names = declared_functions()
     for n in ("launch_kernel", "dpx_frame", "dpx_stage_similarity_graph", "dpx_stage_trivial_cross",
               "dpx_stage_heuristic_cross", "dpx_stage_cells", "dpx_stage_subdivide", "dpx_stage_triangulate",
-              "dpx_stage_export_vbo", "dpx_stage_rasterize", "dpx_stage_components", "dpx_batch_run",
+              "dpx_stage_export_vbo", "dpx_stage_rasterize", "dpx_stage_components",
+              "dpx_stage_border_chains", "dpx_batch_run",
               "dpx_server_submit"):
         assert n in names
 

@@ -236,3 +236,59 @@ def test_batch_result_independent_of_batch_mix(gpu, D):
     spec[2] = (4, False, 0)
     mixed, _ = run_batch(torch, D, frames, spec)
     assert_equal(mixed[2], alone[0], "frame alone vs in a batch")
+
+
+def gpu_border_chains(torch, D, g, max_nodes=None, max_chains=None):
+    H, W = g.shape
+    dev = "cuda:0"
+    max_nodes = 8 * W * H + 8 if max_nodes is None else max_nodes
+    max_chains = W * H + 1 if max_chains is None else max_chains
+    d_g = torch.from_numpy(np.ascontiguousarray(g).reshape(-1)).to(dev)
+    d_scr = torch.zeros(W * H, dtype=torch.uint8, device=dev)
+    d_nodes = torch.full((max(1, max_nodes),), -1, dtype=torch.int32, device=dev)
+    d_sizes = torch.zeros(max(1, max_chains), dtype=torch.int32, device=dev)
+    d_counts = torch.zeros(3, dtype=torch.int32, device=dev)
+    s = torch.cuda.current_stream()
+    D.check(D.lib.dpx_stage_border_chains(d_g.data_ptr(), W, H, d_scr.data_ptr(), d_nodes.data_ptr(), max_nodes,
+                                          d_sizes.data_ptr(), max_chains, d_counts.data_ptr(), s.cuda_stream))
+    torch.cuda.synchronize()
+    nc, nn, status = d_counts.cpu().tolist()
+    nodes, sizes = d_nodes.cpu().numpy(), d_sizes.cpu().numpy()
+    out, k = [], 0
+    for sz in sizes[:nc]:
+        out.append(nodes[k:k + sz].tolist())
+        k += sz
+    return out, nn, status
+
+
+def test_border_chains_match_oracle(gpu, D, O):
+    torch = gpu
+    import json
+    import os
+    from conftest import GOLDEN
+    d = json.load(open(os.path.join(GOLDEN, "reference_graphs.json")))["alex"]
+    alex = np.array(d["graph"], dtype=np.uint8).reshape(d["height"], d["width"])
+    expected = json.load(open(os.path.join(GOLDEN, "alex_chains.json")))["chains"]
+    print("alex", flush=True)
+    got, nn, status = gpu_border_chains(torch, D, alex)
+    assert status == 0 and got == expected and nn == sum(map(len, expected))
+    for kind, seed, W, H in ((0, 31, 256, 224), (2, 32, 97, 61), (4, 33, 120, 90), (0, 34, 1, 1), (2, 35, 7, 1)):
+        print("frame", kind, seed, W, H, flush=True)
+        img = D.synth_frame(kind, seed, W, H, 3)
+        g = O.heuristic_cross(O.trivial_cross(O.similarity_graph(img, W, H)))
+        got, nn, status = gpu_border_chains(torch, D, g)
+        assert status == 0
+        assert got == O.border_chains(g), f"frame {kind}/{seed}"
+
+
+def test_border_chains_report_small_outputs(gpu, D, O):
+    torch = gpu
+    import json
+    import os
+    from conftest import GOLDEN
+    d = json.load(open(os.path.join(GOLDEN, "reference_graphs.json")))["alex"]
+    alex = np.array(d["graph"], dtype=np.uint8).reshape(d["height"], d["width"])
+    _, _, status = gpu_border_chains(torch, D, alex, max_nodes=10)
+    assert status == D.ENOMEM
+    _, _, status = gpu_border_chains(torch, D, alex, max_chains=3)
+    assert status == D.ENOMEM

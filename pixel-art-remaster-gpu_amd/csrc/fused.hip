@@ -1,15 +1,17 @@
 // fused.hip -- the batched path: many frames or sprites of uneven size
 // through two fused kernels.
 //
-//   kf_graph   LDS tile 64x16 (+2 halo): YUV once per pixel, similarity in
-//              four directions (the other four by symmetry), trivial
-//              crossing removal -> snapshot graph (1 B/px)
+//   kf_graph   tile of 64x32 pixels: lane words (YUV fields + class) once
+//              per pixel, similarity in four directions (the other four by
+//              symmetry) with block flags for the trivial crossing removal
+//              -> snapshot graph (1 B/px)
 //   kf_raster  tile of 32x8 cells: heuristic crossing resolution on the
-//              snapshot (tile + halo 3), cell keys (and, with subdivision,
-//              hulls, smoothed polygons and their coverage) in LDS; cells
-//              whose S*S pixels are all their own colour stream them to HBM
-//              at once, the rest are listed and resolved row by row against
-//              the 3x3 candidate cells in draw order.
+//              snapshot (crossed blocks listed and decided once), cell keys
+//              (and, with subdivision, hulls, smoothed polygons and their
+//              coverage) in LDS; cells whose S*S pixels are all their own
+//              colour stream them to HBM at once, the rest are listed and
+//              resolved row by row against the 3x3 candidate cells in draw
+//              order.
 //
 // Coverage of an unsmoothed cell depends only on its 12-bit key and the
 // scale, so it is precomputed once per scale into a table of sample-row

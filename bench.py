@@ -28,6 +28,170 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+def config_items(D, name):
+    """(kind, seed, W, H, channels, flags, scale) of every item of a workload."""
+    if name == "c1":  # 4096 sprites of six sizes, alpha background, 6x
+        out = []
+        for k in range(4096):
+            w, h = D.sprite_size(0, k)
+            out.append((1, k, w, h, 4, D.ALPHA, 6))
+        return out
+    if name == "c2":  # 64 frames of five console sizes, 4x
+        sizes = [(256, 240)] * 13 + [(256, 224)] * 13 + [(320, 224)] * 13 + [(384, 224)] * 13 + [(240, 160)] * 12
+        return [(2, k, w, h, 3, 0, 4) for k, (w, h) in enumerate(sizes)]
+    if name == "c4":  # 16 frames of 640x480, dither and a crossing lattice, 8x
+        return [(4, k, 640, 480, 3, 0, 8) for k in range(16)]
+    raise ValueError(name)
+
+
+def bench_batch_config(args, torch, D):
+    """One batched launch over a whole workload per step; prints one JSON line."""
+    dev = torch.device("cuda:0")
+    stream = torch.cuda.current_stream()
+    spec = config_items(D, args.config)
+    hosts, items = [], []
+    in_off = out_off = 0
+    algo = 0
+    for kind, seed, w, h, ch, flags, s in spec:
+        img = D.synth_frame(kind, seed, w, h, ch)
+        hosts.append(img)
+        items.append(D.Item(in_off, out_off, w, h, img.shape[1], ch, s, 0, flags, w * s * 4))
+        algo += w * h * ch + (w * s) * (h * s) * 4
+        in_off += img.size
+        out_off += (w * s) * (h * s) * 4
+    d_in = torch.from_numpy(np.concatenate([a.reshape(-1) for a in hosts])).to(dev)
+    d_out = torch.empty(out_off, dtype=torch.uint8, device=dev)
+    ctx = D.Context(0)
+    steps = args.steps if args.steps != 200 else 50
+    for _ in range(min(args.warmup, 5)):
+        ctx.run(d_in, d_out, items, stream)
+    torch.cuda.synchronize()
+    ctx.timing(True)
+    ctx.timing_read()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.run(d_in, d_out, items, stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    times = ctx.timing_read()
+    ctx.timing(False)
+    raster_ms = times.raster_ms / max(1, times.batches)
+    graph_ms = times.graph_ms / max(1, times.batches)
+    # parity spot check: first and last items against the plain per-frame path
+    ok = True
+    for k in (0, len(items) - 1):
+        it = items[k]
+        n = it.out_stride * it.height * it.scale
+        got = d_out[it.out_offset:it.out_offset + n].cpu().numpy()
+        ref, _, _ = D.frame(hosts[k], it.width, it.height, it.channels, scale=it.scale, flags=it.flags)
+        ok = ok and bool(np.array_equal(got, ref.reshape(-1)))
+    ctx.close()
+    n_items = len(items)
+    out = {
+        "metric": f"items/s batched ({args.config})", "value": round(n_items * steps / wall, 1), "unit": "items/s",
+        "n_gpus": 1, "steps": steps, "ms_per_step": round(wall * 1e3 / steps, 4), "higher_is_better": True,
+        "dtype": "u8", "data": "synthetic",
+        "config": {"workload": args.config, "items_per_step": n_items, "algorithmic_bytes_per_step": algo},
+        "kernel_ms": {"graph": round(graph_ms, 5), "raster": round(raster_ms, 5)},
+        "roofline": {"bound": "hbm", "achieved": round(algo / (raster_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(algo / (raster_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
+        "parity_spot_check": ok,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def bench_served_streams(args, D, n_streams=128, hz=60.0, seconds=2.5):
+    """128 streams of 320x224 -> 4x frames with Poisson arrivals at 60 Hz each,
+    submitted from host memory to the server; p50/p99 are submit-to-done."""
+    import threading
+    import torch
+    W, H, S = 320, 224, 4
+    # the host link: device -> pinned host copies of one result, and of 64 at once
+    link = {}
+    for label, n in (("one_result", W * S * H * S * 4), ("64_results", 64 * W * S * H * S * 4)):
+        src = torch.empty(n, dtype=torch.uint8, device="cuda:0")
+        dst = torch.empty(n, dtype=torch.uint8, pin_memory=True)
+        for _ in range(3):
+            dst.copy_(src, non_blocking=True)
+        torch.cuda.synchronize()
+        reps = 20
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            dst.copy_(src, non_blocking=True)
+        torch.cuda.synchronize()
+        link[label] = round(n * reps / (time.perf_counter() - t0) / 1e9, 2)
+        del src, dst
+    srv = D.Server(budget_bytes=64 << 30, max_batch=64)
+    sids = [srv.open_stream(W, H, 3, S) for _ in range(n_streams)]
+    # frames and results in pinned host memory from the library: they move by DMA alone
+    frames = []
+    for k in range(64):
+        img = D.synth_frame(0, k, W, H, 3)
+        pinned = D.pinned_empty(img.shape)
+        pinned[:] = img
+        frames.append(pinned)
+    ring = 4
+    outs = [[D.pinned_empty((H * S, W * S * 4)) for _ in range(ring)] for _ in range(n_streams)]
+    # warm-up: a burst that makes the lanes run full batches, so every device
+    # arena has its final size before the timed window opens
+    warm = [srv.submit(sids[k % n_streams], frames[k % len(frames)], outs[k % n_streams][0]) for k in range(256)]
+    for t in warm:
+        srv.wait(t)
+    srv.stats_reset()
+    rng = np.random.default_rng(7)
+    gaps = rng.exponential(1.0 / (hz * n_streams), size=int(hz * n_streams * seconds))
+    arrivals = np.cumsum(gaps)
+    tickets = []
+    lock = threading.Lock()
+    done = threading.Event()
+    failures = []
+
+    def waiter():
+        k = 0
+        while True:
+            with lock:
+                have = k < len(tickets)
+                t = tickets[k] if have else None
+            if not have:
+                if done.is_set():
+                    with lock:
+                        if k >= len(tickets):
+                            return
+                    continue
+                time.sleep(0.0002)
+                continue
+            rc = srv.wait(t)
+            if rc != 0:
+                failures.append(rc)
+            k += 1
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    t0 = time.perf_counter()
+    for k, ta in enumerate(arrivals):
+        while time.perf_counter() - t0 < ta:
+            pass
+        st = k % n_streams
+        t = srv.submit(sids[st], frames[k % len(frames)], outs[st][(k // n_streams) % ring])
+        with lock:
+            tickets.append(t)
+    done.set()
+    th.join()
+    wall = time.perf_counter() - t0
+    s = srv.stats()
+    srv.close()
+    out = {
+        "metric": "served frames (c3): 128 streams of 320x224 -> 4x, Poisson 60 Hz each, host buffers",
+        "value": round(s.completed / wall, 1), "unit": "frames/s", "offered_frames_per_s": hz * n_streams,
+        "n_gpus": 1, "frames": int(s.completed), "failed": int(s.failed) + len(failures),
+        "p50_ms": round(s.p50_ms, 4), "p99_ms": round(s.p99_ms, 4), "mean_batch": round(s.mean_batch, 2),
+        "hbm_peak_bytes": int(s.hbm_peak), "dtype": "u8", "data": "synthetic",
+        "offered_result_GBs": round(hz * n_streams * W * S * H * S * 4 / 1e9, 2), "d2h_link_GBs": link,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -39,6 +203,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference for HBM byte counters)")
+    ap.add_argument("--config", default="c0", choices=["c0", "c1", "c2", "c3", "c4"],
+                    help="c0 (default): the headline 256x224 -> 4x line; c1 4096 alpha sprites -> 6x; "
+                         "c2 64 frames of five console sizes -> 4x; c3 128 served 320x224 streams at 60 Hz; "
+                         "c4 16 640x480 frames -> 8x")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (frames are independent, "
@@ -49,6 +217,10 @@ def main():
     import depixel as D
 
     assert torch.cuda.is_available(), "bench.py needs a GPU"
+    if args.config in ("c1", "c2", "c4"):
+        return bench_batch_config(args, torch, D)
+    if args.config == "c3":
+        return bench_served_streams(args, D)
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
 

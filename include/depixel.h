@@ -164,6 +164,10 @@ typedef struct dpx_context dpx_context;
 
 int dpx_context_create(int device, dpx_context** out);
 void dpx_context_destroy(dpx_context* ctx);
+/* Size the context's device buffers for batches of up to max_items items of
+ * up to max_item_pixels pixels each, and build the tables of `scale` (0: none),
+ * so that dpx_batch_run never has to grow them (growing waits for the device). */
+int dpx_context_reserve(dpx_context* ctx, int max_items, int64_t max_item_pixels, int scale);
 /* Enqueue the batch on `stream`; returns once enqueued.  The context's
  * scratch (graph buffers, descriptors) is reused across calls: calls on one
  * context must be serialised by the caller (same stream or explicit sync). */
@@ -193,7 +197,7 @@ typedef struct dpx_server dpx_server;
 typedef struct {
     uint64_t submitted, completed, failed, rejected, cancelled;
     uint64_t batches;
-    double frames_per_s;     /* completed frames / wall time since create  */
+    double frames_per_s;     /* completed frames / wall time since create or the last reset */
     double p50_ms, p99_ms;   /* submit -> completion latency               */
     double mean_batch;       /* frames per device batch                    */
     uint64_t hbm_in_use, hbm_peak, hbm_budget;
@@ -205,7 +209,9 @@ int dpx_server_open_stream(dpx_server* srv, int width, int height, int channels,
                            const dpx_params* params, int* stream_id);
 int dpx_server_close_stream(dpx_server* srv, int stream_id);
 /* img: host frame (stride bytes/row); out: host RGBA buffer kept alive until
- * the ticket completes. */
+ * the ticket completes.  Frames and results inside blocks from dpx_host_alloc
+ * move by DMA alone; others are staged through pinned buffers (one host copy
+ * each way). */
 int dpx_server_submit(dpx_server* srv, int stream_id, const uint8_t* img, int stride,
                       uint8_t* out, int out_stride, uint64_t* ticket);
 int dpx_server_wait(dpx_server* srv, uint64_t ticket, int timeout_ms);
@@ -213,6 +219,11 @@ int dpx_server_cancel(dpx_server* srv, uint64_t ticket);
 int dpx_server_stats_get(dpx_server* srv, dpx_server_stats* out);
 /* test hook: make the k-th submitted request (0-based) fail on the device side */
 int dpx_server_inject_failure(dpx_server* srv, uint64_t request_index);
+/* restart the counters and latency window of dpx_server_stats_get */
+int dpx_server_stats_reset(dpx_server* srv);
+/* pinned host memory for frames and results (no counterpart in the reference) */
+int dpx_host_alloc(size_t bytes, void** out);
+int dpx_host_free(void* p);
 
 /* ------------------------------------------------------------------------ */
 /* Synthetic inputs (deterministic, splitmix64): the workloads the path is   */

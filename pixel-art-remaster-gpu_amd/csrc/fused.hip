@@ -692,6 +692,53 @@ struct Group {
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
+// Growth of the context's device buffers.  A buffer the device may still be
+// reading is only replaced after the device is idle.
+int grow_graph(dpx_context* ctx, size_t need)
+{
+    if (need <= ctx->g_cap) return DPX_OK;
+    DPX_HIP(hipDeviceSynchronize());
+    (void)hipFree(ctx->d_gaux);
+    (void)hipFree(ctx->d_gfin);
+    ctx->d_gaux = ctx->d_gfin = nullptr;
+    ctx->g_cap = 0;
+    size_t cap = align_up(need + need / 4, 1 << 20);
+    DPX_HIP(hipMalloc(&ctx->d_gaux, cap));
+    DPX_HIP(hipMalloc(&ctx->d_gfin, cap));
+    ctx->g_cap = cap;
+    return DPX_OK;
+}
+
+int grow_desc(dpx_context* ctx, size_t need)
+{
+    if (need <= ctx->desc_cap) return DPX_OK;
+    if (ctx->upload_pending) {
+        DPX_HIP(hipEventSynchronize(ctx->upload_done));
+        ctx->upload_pending = false;
+    }
+    DPX_HIP(hipDeviceSynchronize());
+    (void)hipFree(ctx->d_desc);
+    ctx->d_desc = nullptr;
+    if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
+    ctx->h_pinned = nullptr;
+    ctx->desc_cap = ctx->pinned_cap = 0;
+    ctx->last_desc.clear();
+    size_t cap = align_up(need * 2, 1 << 16);
+    DPX_HIP(hipMalloc(&ctx->d_desc, cap));
+    DPX_HIP(hipHostMalloc(&ctx->h_pinned, cap, hipHostMallocDefault));
+    ctx->desc_cap = cap;
+    ctx->pinned_cap = cap;
+    return DPX_OK;
+}
+
+// Descriptor bytes of a batch of n items: the items twice (all, and per
+// group) and two tile-count prefixes, each array 64-byte aligned, with room
+// for one group per (scale, subdivision) pair.
+size_t desc_bytes(size_t n)
+{
+    return 2 * n * sizeof(DevItem) + 2 * (n + 1) * sizeof(uint32_t) + 64 * (2 * 2 * (kMaxScale + 1) + 4);
+}
+
 // Tiles per item when all items of a prefix have the same count, else 0.
 // Launch grid of a kernel with TX x TY tiles over `dev`: (tile columns, tile
 // rows, items) when all items have the same tile grid, else 1-D over the
@@ -747,6 +794,19 @@ extern "C" void dpx_context_destroy(dpx_context* ctx)
     if (ctx->upload_done) (void)hipEventDestroy(ctx->upload_done);
     for (auto e : ctx->ev_pool) (void)hipEventDestroy(e);
     delete ctx;
+}
+
+extern "C" int dpx_context_reserve(dpx_context* ctx, int max_items, int64_t max_item_pixels, int scale)
+{
+    DPX_CHECK(ctx && max_items > 0 && max_item_pixels > 0, DPX_EINVAL, "bad arguments");
+    DPX_CHECK(scale >= 0 && scale <= kMaxScale, DPX_EINVAL, "scale %d outside 0..%d", scale, kMaxScale);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    DPX_HIP(hipSetDevice(ctx->device));
+    if (int rc = grow_graph(ctx, (size_t)max_items * align_up((size_t)max_item_pixels, 256))) return rc;
+    if (int rc = grow_desc(ctx, desc_bytes((size_t)max_items))) return rc;
+    if (scale > 0)
+        if (int rc = ensure_mask(ctx, scale)) return rc;
+    return DPX_OK;
 }
 
 extern "C" int dpx_batch_graph(dpx_context* ctx, const uint8_t** d_graph, const uint64_t** offsets_host)
@@ -816,17 +876,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         if (int rc = ensure_mask(ctx, kv.second.scale)) return rc;
 
     // graph arenas (grown only when a batch needs more)
-    if (goff > ctx->g_cap) {
-        DPX_HIP(hipDeviceSynchronize());
-        (void)hipFree(ctx->d_gaux);
-        (void)hipFree(ctx->d_gfin);
-        ctx->d_gaux = ctx->d_gfin = nullptr;
-        ctx->g_cap = 0;
-        size_t cap = align_up(goff + goff / 4, 1 << 20);
-        DPX_HIP(hipMalloc(&ctx->d_gaux, cap));
-        DPX_HIP(hipMalloc(&ctx->d_gfin, cap));
-        ctx->g_cap = cap;
-    }
+    if (int rc = grow_graph(ctx, goff)) return rc;
 
     // descriptor blob: items | gprefix | per group (items, prefix)
     std::vector<uint8_t> blob;
@@ -849,18 +899,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
             DPX_HIP(hipEventSynchronize(ctx->upload_done));
             ctx->upload_pending = false;
         }
-        if (blob.size() > ctx->desc_cap) {
-            DPX_HIP(hipDeviceSynchronize());
-            (void)hipFree(ctx->d_desc);
-            ctx->d_desc = nullptr;
-            if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
-            ctx->h_pinned = nullptr;
-            size_t cap = align_up(blob.size() * 2, 1 << 16);
-            DPX_HIP(hipMalloc(&ctx->d_desc, cap));
-            DPX_HIP(hipHostMalloc(&ctx->h_pinned, cap, hipHostMallocDefault));
-            ctx->desc_cap = cap;
-            ctx->pinned_cap = cap;
-        }
+        if (int rc = grow_desc(ctx, blob.size())) return rc;
         memcpy(ctx->h_pinned, blob.data(), blob.size());
         DPX_HIP(hipMemcpyAsync(ctx->d_desc, ctx->h_pinned, blob.size(), hipMemcpyHostToDevice, stream));
         DPX_HIP(hipEventRecord(ctx->upload_done, stream));

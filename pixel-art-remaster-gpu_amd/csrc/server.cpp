@@ -3,18 +3,21 @@
 // The server reserves its HBM budget as one device arena when it is created.
 // Requests (one host frame each, from any number of open streams) queue in
 // arrival order; a request is admitted when its input and output buffers can
-// be carved out of the arena (first fit, oldest first).  Two lanes, each with
-// its own HIP stream, batched-path context and pinned staging buffers, take
-// admitted work whenever they are free: a lane forms a batch of up to
+// be carved out of the arena (first fit, oldest first).  kLanes lanes, each
+// with its own HIP stream, batched-path context and pinned staging buffers,
+// take admitted work whenever they are free: a lane forms a batch of up to
 // max_batch requests, uploads the frames, runs the fused kernel chain,
 // downloads the results and completes the tickets, releasing their arena
-// ranges.  While one lane's batch is on the device the other admits the
+// ranges.  While one lane's batch is on the device the others admit the
 // requests that arrived meanwhile, so requests overlap on the GPU as memory
-// frees up.  Each frame's result depends only on that frame (the kernels keep
-// no state across frames), so it is the same alone and in any mix.
+// frees up.  Opening a stream sizes every lane's context for full batches of
+// its frames, so serving never waits for a context to grow.  Each frame's
+// result depends only on that frame (the kernels keep no state across
+// frames), so it is the same alone and in any mix.
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -29,6 +32,8 @@
 namespace {
 
 using Clock = std::chrono::steady_clock;
+
+constexpr int kLanes = 4;
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
@@ -132,12 +137,28 @@ struct dpx_server {
     uint64_t submitted = 0, completed = 0, failed = 0, rejected = 0, cancelled = 0, batches = 0, batched_frames = 0;
     std::vector<double> latencies_ms;
     Clock::time_point t_create;
-    Lane lanes[2];
+    std::atomic<int64_t> reserved_pixels{ 0 };  // the largest stream frame the lanes are sized for
+    Lane lanes[kLanes];
 };
 
 namespace {
 
 using namespace dpx;
+
+// Pinned host blocks handed out by dpx_host_alloc: frames and results that
+// live in them are copied straight between them and the device.
+std::mutex g_pin_mu;
+std::map<uintptr_t, size_t> g_pinned;  // base -> bytes
+
+bool in_pinned_block(const void* p, size_t n)
+{
+    std::lock_guard<std::mutex> lock(g_pin_mu);
+    auto it = g_pinned.upper_bound(reinterpret_cast<uintptr_t>(p));
+    if (it == g_pinned.begin()) return false;
+    --it;
+    uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    return a >= it->first && a + n <= it->first + it->second;
+}
 
 int grow_pinned(uint8_t** p, size_t* cap, size_t need)
 {
@@ -152,26 +173,36 @@ int grow_pinned(uint8_t** p, size_t* cap, size_t need)
 }
 
 // Runs one admitted batch on a lane; returns a status for the whole batch.
+// Frames and results in blocks from dpx_host_alloc move by DMA alone; others
+// pass through the lane's pinned staging buffers.
 int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
 {
     std::vector<dpx_item> items;
     std::vector<Request*> live;
+    std::vector<char> in_direct, out_direct;
     size_t hin = 0, hout = 0;
     for (Request* r : batch) {
         if (r->inject) continue;
         live.push_back(r);
-        hin += align256(r->in_bytes);
-        hout += align256(r->out_bytes);
+        const size_t row = (size_t)r->W * r->params.scale * 4, rows = (size_t)r->H * r->params.scale;
+        in_direct.push_back(in_pinned_block(r->img, r->in_bytes));
+        out_direct.push_back(in_pinned_block(r->out, (rows - 1) * r->out_stride + row));
+        if (!in_direct.back()) hin += align256(r->in_bytes);
+        if (!out_direct.back()) hout += align256(r->out_bytes);
     }
     if (live.empty()) return DPX_OK;
     if (int rc = grow_pinned(&lane.h_in, &lane.hin_cap, hin)) return rc;
     if (int rc = grow_pinned(&lane.h_out, &lane.hout_cap, hout)) return rc;
     size_t ho = 0;
-    for (Request* r : live) {
-        memcpy(lane.h_in + ho, r->img, r->in_bytes);
-        DPX_HIP(hipMemcpyAsync(srv->d_arena + r->in_off, lane.h_in + ho, r->in_bytes, hipMemcpyHostToDevice,
-                               lane.stream));
-        ho += align256(r->in_bytes);
+    for (size_t k = 0; k < live.size(); k++) {
+        Request* r = live[k];
+        const uint8_t* src = r->img;
+        if (!in_direct[k]) {
+            memcpy(lane.h_in + ho, r->img, r->in_bytes);
+            src = lane.h_in + ho;
+            ho += align256(r->in_bytes);
+        }
+        DPX_HIP(hipMemcpyAsync(srv->d_arena + r->in_off, src, r->in_bytes, hipMemcpyHostToDevice, lane.stream));
         dpx_item it;
         memset(&it, 0, sizeof(it));
         it.in_offset = r->in_off;
@@ -190,7 +221,19 @@ int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
         return rc;
     ho = 0;
     std::vector<size_t> host_off;
-    for (Request* r : live) {
+    for (size_t k = 0; k < live.size(); k++) {
+        Request* r = live[k];
+        const size_t row = (size_t)r->W * r->params.scale * 4, rows = (size_t)r->H * r->params.scale;
+        if (out_direct[k]) {
+            if ((size_t)r->out_stride == row)  // one linear copy
+                DPX_HIP(hipMemcpyAsync(r->out, srv->d_arena + r->out_off, r->out_bytes, hipMemcpyDeviceToHost,
+                                       lane.stream));
+            else
+                DPX_HIP(hipMemcpy2DAsync(r->out, (size_t)r->out_stride, srv->d_arena + r->out_off, row, row, rows,
+                                         hipMemcpyDeviceToHost, lane.stream));
+            host_off.push_back(0);
+            continue;
+        }
         DPX_HIP(hipMemcpyAsync(lane.h_out + ho, srv->d_arena + r->out_off, r->out_bytes, hipMemcpyDeviceToHost,
                                lane.stream));
         host_off.push_back(ho);
@@ -198,10 +241,11 @@ int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
     }
     DPX_HIP(hipStreamSynchronize(lane.stream));
     for (size_t k = 0; k < live.size(); k++) {
+        if (out_direct[k]) continue;
         Request* r = live[k];
-        int row = r->W * r->params.scale * 4, rows = r->H * r->params.scale;
+        const size_t row = (size_t)r->W * r->params.scale * 4, rows = (size_t)r->H * r->params.scale;
         const uint8_t* src = lane.h_out + host_off[k];
-        for (int y = 0; y < rows; y++) memcpy(r->out + (size_t)y * r->out_stride, src + (size_t)y * row, (size_t)row);
+        for (size_t y = 0; y < rows; y++) memcpy(r->out + y * r->out_stride, src + y * row, row);
     }
     return DPX_OK;
 }
@@ -309,7 +353,7 @@ extern "C" int dpx_server_create(int device, uint64_t hbm_budget_bytes, int max_
             return rc;
         }
     }
-    for (int k = 0; k < 2; k++) srv->lanes[k].worker = std::thread(lane_loop, srv, k);
+    for (int k = 0; k < kLanes; k++) srv->lanes[k].worker = std::thread(lane_loop, srv, k);
     *out = srv;
     return DPX_OK;
 }
@@ -344,8 +388,24 @@ extern "C" int dpx_server_open_stream(dpx_server* srv, int width, int height, in
     DPX_CHECK(channels == 3 || channels == 4, DPX_EINVAL, "channels must be 3 or 4");
     DPX_CHECK(params->scale >= 1 && params->scale <= 15, DPX_EINVAL, "scale %d outside 1..15", params->scale);
     DPX_CHECK(params->reserved == 0, DPX_EINVAL, "reserved must be 0");
+    // size every lane for full batches of this stream's frames (outside the
+    // server lock: growing a context waits for the device)
+    const int64_t pixels = (int64_t)width * height;
+    {
+        std::lock_guard<std::mutex> lock(srv->mu);
+        DPX_CHECK(!srv->closing, DPX_ECLOSED, "server closing");
+    }
+    // a batch holds at most as many frames as the HBM budget fits
+    const uint64_t per_frame = align256((size_t)width * channels * height) +
+                               align256((size_t)pixels * params->scale * params->scale * 4);
+    const int items = (int)std::max<uint64_t>(1, std::min<uint64_t>(srv->max_batch, srv->budget / per_frame));
+    for (Lane& lane : srv->lanes)
+        if (int rc = dpx_context_reserve(lane.ctx, items, std::max(pixels, srv->reserved_pixels.load()),
+                                         params->scale))
+            return rc;
     std::lock_guard<std::mutex> lock(srv->mu);
     DPX_CHECK(!srv->closing, DPX_ECLOSED, "server closing");
+    srv->reserved_pixels = std::max(pixels, srv->reserved_pixels.load());
     int id = srv->next_stream++;
     srv->streams[id] = StreamInfo{ width, height, channels, *params, true };
     *stream_id = id;
@@ -457,6 +517,47 @@ extern "C" int dpx_server_stats_get(dpx_server* srv, dpx_server_stats* out)
     out->hbm_in_use = srv->arena.used();
     out->hbm_peak = srv->peak;
     out->hbm_budget = srv->budget;
+    return DPX_OK;
+}
+
+extern "C" int dpx_server_stats_reset(dpx_server* srv)
+{
+    DPX_CHECK(srv, DPX_EINVAL, "null server");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    srv->submitted = srv->completed = srv->failed = srv->rejected = srv->cancelled = 0;
+    srv->batches = srv->batched_frames = 0;
+    srv->latencies_ms.clear();
+    srv->peak = srv->arena.used();
+    srv->t_create = Clock::now();
+    return DPX_OK;
+}
+
+extern "C" int dpx_host_alloc(size_t bytes, void** out)
+{
+    DPX_CHECK(out && bytes > 0, DPX_EINVAL, "bad arguments");
+    *out = nullptr;
+    if (int rc = require_device()) return rc;
+    void* p = nullptr;
+    if (hipHostMalloc(&p, bytes, hipHostMallocDefault) != hipSuccess || !p)
+        return fail(DPX_ENOMEM, "pinned host allocation of %zu bytes failed", bytes);
+    {
+        std::lock_guard<std::mutex> lock(g_pin_mu);
+        g_pinned[reinterpret_cast<uintptr_t>(p)] = bytes;
+    }
+    *out = p;
+    return DPX_OK;
+}
+
+extern "C" int dpx_host_free(void* p)
+{
+    if (!p) return DPX_OK;
+    {
+        std::lock_guard<std::mutex> lock(g_pin_mu);
+        auto it = g_pinned.find(reinterpret_cast<uintptr_t>(p));
+        DPX_CHECK(it != g_pinned.end(), DPX_EINVAL, "not a block from dpx_host_alloc");
+        g_pinned.erase(it);
+    }
+    DPX_HIP(hipHostFree(p));
     return DPX_OK;
 }
 

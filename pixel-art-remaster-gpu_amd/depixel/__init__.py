@@ -98,6 +98,9 @@ _SIGS = {
     "dpx_server_cancel": (c_int, [c_void_p, c_uint64]),
     "dpx_server_stats_get": (c_int, [c_void_p, POINTER(ServerStats)]),
     "dpx_server_inject_failure": (c_int, [c_void_p, c_uint64]),
+    "dpx_server_stats_reset": (c_int, [c_void_p]),
+    "dpx_host_alloc": (c_int, [ctypes.c_size_t, POINTER(c_void_p)]),
+    "dpx_host_free": (c_int, [c_void_p]),
     "dpx_synth_frame": (c_int, [c_int, c_uint64, c_int, c_int, c_int, c_void_p, c_int]),
     "dpx_synth_sprite_size": (c_int, [c_uint64, c_int, POINTER(c_int), POINTER(c_int)]),
 }
@@ -148,6 +151,21 @@ def synth_frame(kind: int, seed: int, width: int, height: int, channels: int = 3
     buf = np.zeros((height, stride), dtype=np.uint8)
     check(lib.dpx_synth_frame(kind, seed, width, height, channels, _ptr(buf), stride))
     return buf
+
+
+def pinned_empty(shape, dtype=np.uint8) -> np.ndarray:
+    """A numpy array in pinned host memory (dpx_host_alloc), freed with the
+    last array that views it.  The server moves frames and results in such
+    arrays by DMA alone."""
+    import weakref
+    dt = np.dtype(dtype)
+    n = int(np.prod(shape)) * dt.itemsize
+    p = c_void_p()
+    check(lib.dpx_host_alloc(n, ctypes.byref(p)))
+    buf = (ctypes.c_uint8 * n).from_address(p.value)
+    fin = weakref.finalize(buf, lib.dpx_host_free, p.value)
+    fin.atexit = False  # at interpreter exit the process releases it
+    return np.frombuffer(buf, dtype=dt).reshape(shape)
 
 
 def sprite_size(seed: int, index: int) -> tuple[int, int]:
@@ -287,6 +305,9 @@ class Server:
         s = ServerStats()
         check(lib.dpx_server_stats_get(self.h, ctypes.byref(s)))
         return s
+
+    def stats_reset(self):
+        check(lib.dpx_server_stats_reset(self.h))
 
 
 def header_symbols() -> list[str]:

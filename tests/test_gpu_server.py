@@ -134,3 +134,26 @@ def test_wait_timeout_returns_while_pending(gpu, D):
     assert rc in (D.ETIMEOUT, 0)
     assert all(srv.wait(t) == 0 for t in tickets if not (t == tickets[-1] and rc == 0))
     srv.close()
+
+
+def test_pinned_frames_and_results_move_by_dma(gpu, D, O):
+    """Frames and results in dpx_host_alloc blocks skip the staging copies;
+    the output is the same, also with a padded result stride."""
+    srv = D.Server(budget_bytes=256 << 20, max_batch=8)
+    W, H, s = 72, 40, 4
+    sid = srv.open_stream(W, H, 3, s)
+    cases = []
+    for k in range(6):
+        img = D.synth_frame(2, 300 + k, W, H, 3)
+        pin = D.pinned_empty(img.shape)
+        pin[:] = img
+        pad = 64 if k % 2 else 0
+        out = D.pinned_empty((H * s, W * s * 4 + pad))
+        out[:] = 7
+        cases.append((img, pin, out, pad, srv.submit(sid, pin, out)))
+    for img, pin, out, pad, t in cases:
+        assert srv.wait(t) == 0
+        assert np.array_equal(out[:, :W * s * 4], expect(O, img, W, H, s))
+        if pad:
+            assert (out[:, W * s * 4:] == 7).all(), "the padding of a result row is not written"
+    srv.close()

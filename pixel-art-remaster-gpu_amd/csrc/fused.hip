@@ -149,32 +149,18 @@ __device__ inline bool simple_cell(uint32_t own, uint32_t upl, uint32_t up, uint
 // work distribution: block -> (item, tile column, tile row).  When every item
 // of a launch has the same tile grid, the launch grid is (tile columns, tile
 // rows, items) and the block index says it all; otherwise the grid is 1-D
-// over all tiles and the item is found by binary search over the prefix of
-// tile counts.
+// over all tiles and a map built on the host gives each block its item and
+// tile: tile_map[b] = (item, tile row << 16 | tile column).
 // ---------------------------------------------------------------------------
 struct TileAt {
     int item, tx, ty;
 };
 
-__device__ inline int find_item(const uint32_t* __restrict__ prefix, int n, uint32_t b)
-{
-    int lo = 0, hi = n;  // prefix[lo] <= b < prefix[hi]
-    while (hi - lo > 1) {
-        int mid = (lo + hi) >> 1;
-        if (prefix[mid] <= b) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
-template <int TX>
-__device__ inline TileAt locate_tile(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix, int n,
-                                     int grid3d)
+__device__ inline TileAt locate_tile(const uint2* __restrict__ tile_map, int grid3d)
 {
     if (grid3d) return TileAt{ (int)blockIdx.z, (int)blockIdx.x, (int)blockIdx.y };
-    int item = find_item(prefix, n, blockIdx.x);
-    int tile = (int)(blockIdx.x - prefix[item]);
-    int tiles_x = (items[item].W + TX - 1) / TX;
-    return TileAt{ item, tile % tiles_x, tile / tiles_x };
+    const uint2 m = tile_map[blockIdx.x];
+    return TileAt{ (int)m.x, (int)(m.y & 0xFFFFu), (int)(m.y >> 16) };
 }
 
 // ---------------------------------------------------------------------------
@@ -198,15 +184,15 @@ constexpr int GLX = GTX + 2, GLY = GTY + 4;  // lane words
 constexpr int GHX = GTX + 1, GHY = GTY + 2;  // positions of pass 2
 constexpr int GHP = GTX + 4;                 // byte pitch of pass 2's rows: column c is x = x0 - 4 + c
 
-__global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix,
-                                                int n_items, int grid3d, const uint8_t* __restrict__ in,
+__global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
+                                                int grid3d, const uint8_t* __restrict__ in,
                                                 uint8_t* __restrict__ gaux)
 {
     __shared__ uint32_t s_lane[GLX * GLY];
     __shared__ uint32_t s_h32[GHY * GHP / 4];
     uint8_t* s_h = reinterpret_cast<uint8_t*>(s_h32);
     const int t = threadIdx.x;
-    const TileAt at = locate_tile<GTX>(items, prefix, n_items, grid3d);
+    const TileAt at = locate_tile(tile_map, grid3d);
     const DevItem it = items[at.item];
     const int x0 = at.tx * GTX, y0 = at.ty * GTY;
 
@@ -293,8 +279,8 @@ struct RasterWin {
 
 // `items` are the items of one (scale, subdivision) group, in launch order.
 template <int S, bool SUB>
-__global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const uint32_t* __restrict__ prefix,
-                                                 int n_items, int grid3d,
+__global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
+                                                 int grid3d,
                                                  const uint8_t* __restrict__ in, const uint8_t* __restrict__ gaux,
                                                  uint8_t* __restrict__ gfin, const int8_t* __restrict__ hull_table,
                                                  const uint32_t* __restrict__ mask_table,
@@ -318,7 +304,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
 
     const int t = threadIdx.x;
-    const TileAt at = locate_tile<RTX>(items, prefix, n_items, grid3d);
+    const TileAt at = locate_tile(tile_map, grid3d);
     const DevItem it = items[at.item];
     const int x0 = at.tx * RTX, y0 = at.ty * RTY;
     const uint8_t* aux = gaux + it.g_off;
@@ -607,7 +593,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     }
 }
 
-using RasterFn = void (*)(const DevItem*, const uint32_t*, int, int, const uint8_t*, const uint8_t*, uint8_t*,
+using RasterFn = void (*)(const DevItem*, const uint2*, int, const uint8_t*, const uint8_t*, uint8_t*,
                           const int8_t*, const uint32_t*, const uint16_t*, uint8_t*);
 
 template <int S>
@@ -687,7 +673,9 @@ struct Group {
     int scale;
     bool sub;
     std::vector<DevItem> dev;  // the group's items, in launch order
-    std::vector<uint32_t> prefix;
+    std::vector<uint2> map;    // tile map of a 1-D launch
+    dim3 grid;
+    int grid3d = 0;
 };
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
@@ -731,27 +719,37 @@ int grow_desc(dpx_context* ctx, size_t need)
     return DPX_OK;
 }
 
-// Descriptor bytes of a batch of n items: the items twice (all, and per
-// group) and two tile-count prefixes, each array 64-byte aligned, with room
-// for one group per (scale, subdivision) pair.
+// Descriptor bytes of a batch of n items whose items share a tile grid: the
+// items twice (all, and per group), each array 64-byte aligned, with room for
+// one group per (scale, subdivision) pair.  (Batches of mixed sizes add tile
+// maps and may grow the buffer.)
 size_t desc_bytes(size_t n)
 {
-    return 2 * n * sizeof(DevItem) + 2 * (n + 1) * sizeof(uint32_t) + 64 * (2 * 2 * (kMaxScale + 1) + 4);
+    return 2 * n * sizeof(DevItem) + 64 * (2 * 2 * (kMaxScale + 1) + 4);
 }
 
 // Tiles per item when all items of a prefix have the same count, else 0.
 // Launch grid of a kernel with TX x TY tiles over `dev`: (tile columns, tile
-// rows, items) when all items have the same tile grid, else 1-D over the
-// prefix of tile counts.  *grid3d tells the kernel which.
+// rows, items) when all items have the same tile grid, else 1-D over all
+// tiles with a map from block to (item, tile row << 16 | tile column).
+// *grid3d tells the kernel which.  Tile columns and rows fit 16 bits: items
+// are at most 32768 pixels wide and high.
 template <int TX, int TY>
-dim3 tile_grid(const std::vector<DevItem>& dev, const std::vector<uint32_t>& prefix, int* grid3d)
+dim3 tile_grid(const std::vector<DevItem>& dev, int* grid3d, std::vector<uint2>* map)
 {
-    const unsigned tx = (unsigned)(dev[0].W + TX - 1) / TX, ty = (unsigned)(dev[0].H + TY - 1) / TY;
-    bool same = dev.size() <= 65535 && ty <= 65535;
+    const unsigned tx0 = (unsigned)(dev[0].W + TX - 1) / TX, ty0 = (unsigned)(dev[0].H + TY - 1) / TY;
+    bool same = dev.size() <= 65535;
     for (const DevItem& d : dev)
-        same = same && (unsigned)(d.W + TX - 1) / TX == tx && (unsigned)(d.H + TY - 1) / TY == ty;
+        same = same && (unsigned)(d.W + TX - 1) / TX == tx0 && (unsigned)(d.H + TY - 1) / TY == ty0;
+    map->clear();
     *grid3d = same ? 1 : 0;
-    return same ? dim3(tx, ty, (unsigned)dev.size()) : dim3(prefix.back(), 1, 1);
+    if (same) return dim3(tx0, ty0, (unsigned)dev.size());
+    for (size_t k = 0; k < dev.size(); k++) {
+        const unsigned tx = (unsigned)(dev[k].W + TX - 1) / TX, ty = (unsigned)(dev[k].H + TY - 1) / TY;
+        for (unsigned y = 0; y < ty; y++)
+            for (unsigned x = 0; x < tx; x++) map->push_back(make_uint2((unsigned)k, y << 16 | x));
+    }
+    return dim3((unsigned)map->size(), 1, 1);
 }
 
 }  // namespace
@@ -827,7 +825,6 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
 
     // plan: device items, graph tiles, raster groups by (scale, subdivide)
     std::vector<DevItem> dev(n_items);
-    std::vector<uint32_t> gprefix(n_items + 1, 0);
     ctx->g_offsets.assign(n_items, 0);
     std::map<int, Group> groups;
     uint64_t goff = 0;
@@ -861,16 +858,11 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         d.graph_w4 = a.width % 4 == 0;
         ctx->g_offsets[k] = goff;
         goff += align_up((size_t)a.width * a.height, 256);
-        uint32_t gt = (uint32_t)(((a.width + GTX - 1) / GTX) * ((a.height + GTY - 1) / GTY));
-        gprefix[k + 1] = gprefix[k] + gt;
         int gkey = a.scale * 2 + d.subdivide;
         Group& grp = groups[gkey];
         grp.scale = a.scale;
         grp.sub = d.subdivide;
-        if (grp.prefix.empty()) grp.prefix.push_back(0);
         grp.dev.push_back(d);
-        uint32_t rt = (uint32_t)(((a.width + RTX - 1) / RTX) * ((a.height + RTY - 1) / RTY));
-        grp.prefix.push_back(grp.prefix.back() + rt);
     }
     for (auto& kv : groups)
         if (int rc = ensure_mask(ctx, kv.second.scale)) return rc;
@@ -878,20 +870,26 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
     // graph arenas (grown only when a batch needs more)
     if (int rc = grow_graph(ctx, goff)) return rc;
 
-    // descriptor blob: items | gprefix | per group (items, prefix)
+    // launch grids
+    int g3d = 0;
+    std::vector<uint2> gmap;
+    const dim3 ggrid = tile_grid<GTX, GTY>(dev, &g3d, &gmap);
+    for (auto& kv : groups) kv.second.grid = tile_grid<RTX, RTY>(kv.second.dev, &kv.second.grid3d, &kv.second.map);
+
+    // descriptor blob: items | graph tile map | per group (items, tile map)
     std::vector<uint8_t> blob;
     auto put = [&](const void* p, size_t n) {
         size_t off = align_up(blob.size(), 64);
         blob.resize(off + n);
-        memcpy(blob.data() + off, p, n);
+        if (n) memcpy(blob.data() + off, p, n);
         return off;
     };
     size_t off_items = put(dev.data(), dev.size() * sizeof(DevItem));
-    size_t off_gprefix = put(gprefix.data(), gprefix.size() * sizeof(uint32_t));
+    size_t off_gmap = put(gmap.data(), gmap.size() * sizeof(uint2));
     std::vector<std::pair<size_t, size_t>> goffs;
     for (auto& kv : groups) {
         size_t a = put(kv.second.dev.data(), kv.second.dev.size() * sizeof(DevItem));
-        size_t b = put(kv.second.prefix.data(), kv.second.prefix.size() * sizeof(uint32_t));
+        size_t b = put(kv.second.map.data(), kv.second.map.size() * sizeof(uint2));
         goffs.push_back({ a, b });
     }
     if (blob != ctx->last_desc) {
@@ -907,7 +905,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         ctx->last_desc.swap(blob);
     }
     const DevItem* d_items = reinterpret_cast<const DevItem*>(ctx->d_desc + off_items);
-    const uint32_t* d_gprefix = reinterpret_cast<const uint32_t*>(ctx->d_desc + off_gprefix);
+    const uint2* d_gmap = reinterpret_cast<const uint2*>(ctx->d_desc + off_gmap);
 
     hipEvent_t ev[3] = {};
     if (ctx->timing) {
@@ -920,10 +918,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         ctx->ev_used += 3;
         DPX_HIP(hipEventRecord(ev[0], stream));
     }
-    int grid3d = 0;
-    dim3 ggrid = tile_grid<GTX, GTY>(dev, gprefix, &grid3d);
-    hipLaunchKernelGGL(kf_graph, ggrid, dim3(256), 0, stream, d_items, d_gprefix, n_items, grid3d, d_in,
-                       ctx->d_gaux);
+    hipLaunchKernelGGL(kf_graph, ggrid, dim3(256), 0, stream, d_items, d_gmap, g3d, d_in, ctx->d_gaux);
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[1], stream));
     int gk = 0;
     for (auto& kv : groups) {
@@ -931,11 +926,9 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         RasterFn fn = pick_raster(grp.scale, grp.sub);
         DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", grp.scale);
         const DevItem* d_gitems = reinterpret_cast<const DevItem*>(ctx->d_desc + goffs[gk].first);
-        const uint32_t* d_rp = reinterpret_cast<const uint32_t*>(ctx->d_desc + goffs[gk].second);
-        dim3 rgrid = tile_grid<RTX, RTY>(grp.dev, grp.prefix, &grid3d);
-        hipLaunchKernelGGL(fn, rgrid, dim3(256), 0, stream, d_gitems, d_rp, (int)grp.dev.size(), grid3d, d_in,
-                           ctx->d_gaux, ctx->d_gfin, ctx->d_hull, ctx->d_mask[grp.scale], ctx->d_info[grp.scale],
-                           d_out);
+        const uint2* d_rmap = reinterpret_cast<const uint2*>(ctx->d_desc + goffs[gk].second);
+        hipLaunchKernelGGL(fn, grp.grid, dim3(256), 0, stream, d_gitems, d_rmap, grp.grid3d, d_in, ctx->d_gaux,
+                           ctx->d_gfin, ctx->d_hull, ctx->d_mask[grp.scale], ctx->d_info[grp.scale], d_out);
         gk++;
     }
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[2], stream));

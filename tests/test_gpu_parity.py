@@ -292,3 +292,18 @@ def test_border_chains_report_small_outputs(gpu, D, O):
     assert status == D.ENOMEM
     _, _, status = gpu_border_chains(torch, D, alex, max_chains=3)
     assert status == D.ENOMEM
+
+
+def test_large_same_size_batch_matches_oracle(gpu, D, O):
+    """150 items of one size (a 3-D launch grid over items) with alternating
+    row orders: every item equals the oracle (graph and pixels)."""
+    torch = gpu
+    n, W, H, S = 150, 40, 24, 3
+    frames = [(D.synth_frame(2 if k % 2 else 0, 500 + k, W, H, 3), W, H, 3) for k in range(n)]
+    spec = [(S, False, D.TOPDOWN if k % 3 == 0 else 0) for k in range(n)]
+    outs, graphs = run_batch(torch, D, frames, spec)
+    for k in range(n):
+        img = frames[k][0]
+        ref = O.depixelize(img, W, H, scale=S, flags=spec[k][2])
+        assert_equal(graphs[k], ref["graph"], f"item {k} graph")
+        assert_equal(outs[k], ref["rgba"], f"item {k} rgba")

@@ -14,8 +14,10 @@
  *   - image buffers: 8-bit pixels, `channels` 3 (the reference's BGR order
  *     unless DPX_RGB) or 4 (BGRA/RGBA, alpha extension), `stride` bytes per
  *     row, rows bottom-up (the reference's flipped OpenCV layout,
- *     Image.cpp:21-38) unless DPX_TOPDOWN;
- *   - output images: RGBA8, (scale*W) x (scale*H), same row order as the input;
+ *     Image.cpp:21-38) unless DPX_TOPDOWN; on the batched and serving paths
+ *     also `channels` 1 with DPX_INDEXED (one palette index per pixel);
+ *   - output images: RGBA8, (scale*W) x (scale*H), same row order as the
+ *     input unless DPX_FLIP_OUT;
  *   - graph: 1 byte per pixel, bit e = similar to neighbour e
  *     (e -> (dx,dy): 0:(-1,+1) 1:(0,+1) 2:(+1,+1) 3:(-1,0) 4:(+1,0)
  *      5:(-1,-1) 6:(0,-1) 7:(+1,-1); +1 = next row up);
@@ -43,6 +45,13 @@ typedef struct { uint8_t x, y, z, w; } dpx_uchar4;
 #define DPX_RGB      1u    /* byte order R,G,B(,A) instead of B,G,R(,A)              */
 #define DPX_TOPDOWN  2u    /* buffer row 0 is the top row (output follows the input) */
 #define DPX_ALPHA    4u    /* channels==4: alpha-0 pixels are their own colour class */
+#define DPX_FLIP_OUT 8u    /* output rows in the opposite order to the input's       */
+/* channels==1: each pixel byte indexes a palette of 256 four-byte entries
+ * (c0,c1,c2,a in the pixel byte order DPX_RGB selects; a is the alpha an
+ * RGBA pixel would carry).  An indexed frame gives exactly the output of the
+ * 4-channel frame made of its palette entries.  Batched path (palette at
+ * dpx_item.palette_offset) and serving path (dpx_server_set_palette) only. */
+#define DPX_INDEXED  16u
 
 typedef enum {
     DPX_OK = 0,
@@ -60,7 +69,8 @@ typedef enum {
 typedef struct {
     int32_t scale;       /* output pixels per input pixel, 1..32 (batched path: 1..15) */
     int32_t subdivide;   /* corner-cutting smoothing on/off (simpleVBO.cpp:41)       */
-    uint32_t flags;      /* DPX_RGB | DPX_TOPDOWN | DPX_ALPHA                         */
+    uint32_t flags;      /* DPX_RGB | DPX_TOPDOWN | DPX_ALPHA | DPX_FLIP_OUT
+                            (| DPX_INDEXED on the serving path)                       */
     int32_t reserved;    /* must be 0                                                 */
 } dpx_params;
 
@@ -145,6 +155,38 @@ int dpx_stage_border_chains(const uint8_t* d_graph, int width, int height, uint8
                             int32_t* d_nodes, int max_nodes, int32_t* d_sizes, int max_chains,
                             int32_t* d_counts, void* stream);
 
+/* Presentation: what the reference's GLUT window shows (display()
+ * callbacks.cpp:39-72 drawing renderCuda simpleVBO.cpp:185-440 into the
+ * 1200x800 orthographic window of main.cpp:19-20, 260-276), as an RGBA8
+ * window image.  Window pixel (X, Y), Y = 0 the bottom row, samples the
+ * image point ((X + ox - tx) / s, (Y + oy - ty) / s) with s = scale_x100/100
+ * and (tx, ty) the translation: one sample at (ox, oy) = (1/2, 1/2), or with
+ * DPX_VIEW_AA four samples at (3/8, 1/8), (7/8, 3/8), (1/8, 5/8), (5/8, 7/8)
+ * averaged per channel, rounding halves up (GL_MULTISAMPLE's sample pattern
+ * is driver-defined; this is the standard rotated 4x grid).  A sample takes
+ * the colour of the last cell in draw order whose triangles cover it (the
+ * rasterize stage's rule), or the clear colour (0,0,0,255) (main.cpp:260).
+ * DPX_VIEW_QUADS draws each pixel as its s x s square instead (drawMode
+ * GL_QUADS, key 'q', simpleVBO.cpp:288-331); d_diagram and d_edge_count may
+ * then be NULL.  With scale_x100 = 100 * scale, no translation, a window of
+ * (scale*W) x (scale*H) and no AA, the cells mode equals dpx_stage_rasterize. */
+#define DPX_VIEW_CELLS    0   /* drawMode VBO_RENDER, the default (callbacks.cpp:25, key 'p') */
+#define DPX_VIEW_QUADS    1   /* drawMode GL_QUADS (key 'q')                                  */
+#define DPX_VIEW_AA       1u  /* GL_MULTISAMPLE on (simpleVBO.cpp:238-250, key 'a')           */
+#define DPX_VIEW_TOPDOWN  2u  /* window row 0 written first is the top row                    */
+typedef struct {
+    int32_t mode;                      /* DPX_VIEW_CELLS or DPX_VIEW_QUADS                        */
+    int32_t scale_x100;                /* glScalef factor in hundredths, 25..3200 (simpleVBO.cpp:43
+                                          default 8.0 = 800; keys step 0.5, mouse drag 0.01)      */
+    int32_t translate_x, translate_y;  /* window pixels, -65536..65536 (callbacks.cpp:32-33: 1, 1) */
+    int32_t width, height;             /* window, 1..16384 (main.cpp:19-20: 1200 x 800)          */
+    uint32_t flags;                    /* DPX_VIEW_AA | DPX_VIEW_TOPDOWN                          */
+    int32_t reserved;                  /* must be 0                                               */
+} dpx_view;
+int dpx_stage_present(const uint8_t* d_img, int width, int height, int stride, int channels,
+                      uint32_t flags, const dpx_point* d_diagram, const int32_t* d_edge_count,
+                      const dpx_view* view, uint8_t* d_out, int out_stride, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Batched path: many frames / sprites of uneven sizes through one fused     */
 /* kernel chain.  Inputs and outputs live in caller-owned device arenas.     */
@@ -158,6 +200,8 @@ typedef struct {
     int32_t subdivide;
     uint32_t flags;
     int32_t out_stride;   /* bytes per output row (>= 4*scale*width)           */
+    uint64_t palette_offset;  /* DPX_INDEXED: byte offset of the 1 KiB palette
+                                 in d_in, a multiple of 4; otherwise ignored  */
 } dpx_item;
 
 typedef struct dpx_context dpx_context;
@@ -208,6 +252,11 @@ void dpx_server_destroy(dpx_server* srv);
 int dpx_server_open_stream(dpx_server* srv, int width, int height, int channels,
                            const dpx_params* params, int* stream_id);
 int dpx_server_close_stream(dpx_server* srv, int stream_id);
+/* The palette of a DPX_INDEXED stream (channels 1): 256 four-byte entries,
+ * copied by the server.  A frame is rendered with the palette set when it was
+ * submitted (1 KiB that travels to the device with the frame).  Frames of an
+ * indexed stream are refused until a palette is set. */
+int dpx_server_set_palette(dpx_server* srv, int stream_id, const uint8_t* palette);
 /* img: host frame (stride bytes/row); out: host RGBA buffer kept alive until
  * the ticket completes.  Frames and results inside blocks from dpx_host_alloc
  * move by DMA alone; others are staged through pinned buffers (one host copy

@@ -177,6 +177,18 @@ def rasterize(img, W, H, diagram, ec, scale, channels=3, flags=0):
     return out.reshape(H * scale, W * scale, 4)
 
 
+def present(img, W, H, diagram, ec, mode=0, scale_x100=800, tx=1, ty=1, width=1200, height=800, view_flags=1,
+            channels=3, flags=0):
+    """The viewer window (defaults: the reference's 1200x800 window, scale 8, translate (1,1), AA on);
+    returns (height, width, 4) RGBA in the requested row order."""
+    out = np.zeros((height, width * 4), dtype=np.uint8)
+    d = None if diagram is None else np.ascontiguousarray(diagram)
+    e = None if ec is None else np.ascontiguousarray(ec)
+    _ok(lib().orc_present(_p(img), W, H, img.shape[1], channels, flags, _p(d), _p(e), mode, scale_x100, tx, ty,
+                          width, height, view_flags, _p(out), out.shape[1]), "present")
+    return out.reshape(height, width, 4)
+
+
 def depixelize(img, W, H, scale=4, subdivide=False, channels=3, flags=0, want_diagram=False):
     """Full per-frame sequence; returns dict(rgba, graph, edge_count[, diagram], tri_failed)."""
     out = np.zeros((H * scale, W * scale * 4), dtype=np.uint8)

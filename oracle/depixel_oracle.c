@@ -918,6 +918,116 @@ int orc_rasterize(const uint8_t* img, int W, int H, int stride, int channels, in
 }
 
 /* ------------------------------------------------------------------------- */
+/* F4  presentation in the GLUT window                                        */
+/*   display() callbacks.cpp:39-72: clear to (0,0,0,1) (main.cpp:260),        */
+/*   modelview translate (tx, ty) (callbacks.cpp:32-33, 47), orthographic      */
+/*   1:1 window (main.cpp:276), then renderCuda simpleVBO.cpp:185-440:         */
+/*   VBO_RENDER draws every cell's triangles with glScalef(s) (258, 281);      */
+/*   GL_QUADS draws pixel (j,i) as the triangles (js,is),(js,is+s),(js+s,is+s) */
+/*   and (js+s,is+s),(js+s,is),(js,is) in its colour (simpleVBO.cpp:295-321).  */
+/*   Definition used by the build (GL's sampling is closed, parity unpinned):  */
+/*   - window pixel (X,Y), Y=0 the bottom row, has one sample at offset        */
+/*     (1/2,1/2), or with antialiasing four at (3,1),(7,3),(1,5),(5,7) / 8,    */
+/*     resolved per channel as (sum + 2) / 4;                                  */
+/*   - a sample is covered by the coverage rule of orc_rasterize, primitives   */
+/*     in draw order, later ones overwrite; s = scale_x100 / 100.              */
+/*   Coordinates: image point p is p * 512 * scale_x100 integer units; a       */
+/*   window sample (X + ox/8) maps to (8 (X - tx) + ox) * 6400 units.          */
+/*   Pinned by: equality with orc_rasterize for integer scales without         */
+/*   translation or antialiasing; nearest-neighbour closed form of GL_QUADS;   */
+/*   brute force over exact rationals (tests/test_oracle_present.py).         */
+/* ------------------------------------------------------------------------- */
+
+static void present_triangle(long long ax, long long ay, long long bx, long long by, long long cx, long long cy,
+                             uint32_t rgba, int tx, int ty, int ww, int wh, int ns, const int* ox, const int* oy,
+                             uint32_t* samp)
+{
+    long long o = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+    if (o == 0) return;
+    if (o < 0) { long long t1 = bx, t2 = by; bx = cx; by = cy; cx = t1; cy = t2; }
+    long long mnx = ax < bx ? (ax < cx ? ax : cx) : (bx < cx ? bx : cx);
+    long long mxx = ax > bx ? (ax > cx ? ax : cx) : (bx > cx ? bx : cx);
+    long long mny = ay < by ? (ay < cy ? ay : cy) : (by < cy ? by : cy);
+    long long mxy = ay > by ? (ay > cy ? ay : cy) : (by > cy ? by : cy);
+    long long x0 = (long long)floor(((double)mnx / 6400.0 - 8.0) / 8.0) + tx - 1;
+    long long x1 = (long long)ceil((double)mxx / 6400.0 / 8.0) + tx + 1;
+    long long y0 = (long long)floor(((double)mny / 6400.0 - 8.0) / 8.0) + ty - 1;
+    long long y1 = (long long)ceil((double)mxy / 6400.0 / 8.0) + ty + 1;
+    if (x0 < 0) x0 = 0;
+    if (y0 < 0) y0 = 0;
+    if (x1 > ww - 1) x1 = ww - 1;
+    if (y1 > wh - 1) y1 = wh - 1;
+    for (long long Y = y0; Y <= y1; Y++)
+        for (long long X = x0; X <= x1; X++)
+            for (int s = 0; s < ns; s++) {
+                long long sx = (8 * (X - tx) + ox[s]) * 6400, sy = (8 * (Y - ty) + oy[s]) * 6400;
+                if (edge_in(ax, ay, bx, by, sx, sy) && edge_in(bx, by, cx, cy, sx, sy) &&
+                    edge_in(cx, cy, ax, ay, sx, sy))
+                    samp[((size_t)Y * ww + (size_t)X) * ns + s] = rgba;
+            }
+}
+
+int orc_present(const uint8_t* img, int W, int H, int stride, int channels, int flags, const orc_pt* diagram,
+                const int32_t* edge_count, int mode, int scale_x100, int tx, int ty, int win_w, int win_h,
+                int view_flags, uint8_t* out, int out_stride)
+{
+    if (!img || !out || W <= 0 || H <= 0 || win_w <= 0 || win_h <= 0 || scale_x100 <= 0) return ORC_EINVAL;
+    if (mode != 0 && mode != 1) return ORC_EINVAL;
+    if (mode == 0 && (!diagram || !edge_count)) return ORC_EINVAL;
+    orc_image im = { img, W, H, stride, channels, flags };
+    const int aa = view_flags & 1, ns = aa ? 4 : 1;
+    const int sox[4] = { 3, 7, 1, 5 }, soy[4] = { 1, 3, 5, 7 }, cox[1] = { 4 }, coy[1] = { 4 };
+    const int* ox = aa ? sox : cox;
+    const int* oy = aa ? soy : coy;
+    size_t n = (size_t)win_w * win_h * ns;
+    uint32_t* samp = (uint32_t*)malloc(n * sizeof(uint32_t));
+    if (!samp) return ORC_EINVAL;
+    for (size_t k = 0; k < n; k++) samp[k] = 0xFF000000u;
+    const long long sc = scale_x100, unit = 512LL * sc;
+    int bad = 0;
+    for (int j = 0; j < H; j++)
+        for (int i = 0; i < W; i++) {
+            uint32_t rgba = cell_rgba(&im, i, j);
+            if (mode == 1) {
+                long long x0 = i * unit, x1 = (i + 1) * unit, y0 = j * unit, y1 = (j + 1) * unit;
+                present_triangle(x0, y0, x0, y1, x1, y1, rgba, tx, ty, win_w, win_h, ns, ox, oy, samp);
+                present_triangle(x1, y1, x1, y0, x0, y0, rgba, tx, ty, win_w, win_h, ns, ox, oy, samp);
+                continue;
+            }
+            size_t idx = (size_t)j * W + i;
+            int ntri = edge_count[idx] - 2;
+            for (int t = 0; t < ntri && t < CS / 3; t++) {
+                const orc_pt* v = &diagram[idx * CS + (size_t)t * 3];
+                long long u[6];
+                for (int q = 0; q < 3; q++) {
+                    double qx = (double)v[q].x * 64.0, qy = (double)v[q].y * 64.0;
+                    if (floor(qx) != qx || floor(qy) != qy) bad++;
+                    u[2 * q] = (long long)floor(qx) * 8 * sc;
+                    u[2 * q + 1] = (long long)floor(qy) * 8 * sc;
+                }
+                present_triangle(u[0], u[1], u[2], u[3], u[4], u[5], rgba, tx, ty, win_w, win_h, ns, ox, oy, samp);
+            }
+        }
+    for (int Y = 0; Y < win_h; Y++)
+        for (int X = 0; X < win_w; X++) {
+            const uint32_t* s = &samp[((size_t)Y * win_w + X) * ns];
+            uint32_t c = s[0];
+            if (ns == 4) {
+                c = 0;
+                for (int b = 0; b < 4; b++) {
+                    uint32_t sum = 0;
+                    for (int k = 0; k < 4; k++) sum += (s[k] >> (8 * b)) & 0xFFu;
+                    c |= ((sum + 2) >> 2) << (8 * b);
+                }
+            }
+            int row = (view_flags & 2) ? win_h - 1 - Y : Y;
+            memcpy(out + (size_t)row * out_stride + (size_t)X * 4, &c, 4);
+        }
+    free(samp);
+    return bad ? ORC_EINVAL : ORC_OK;
+}
+
+/* ------------------------------------------------------------------------- */
 /* A1  the per-frame sequence of launch_kernel (kernel.cu:286-526)            */
 /* ------------------------------------------------------------------------- */
 

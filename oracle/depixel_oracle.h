@@ -68,6 +68,13 @@ int orc_export_vbo(const uint8_t* img, int W, int H, int stride, int channels, i
 int orc_rasterize(const uint8_t* img, int W, int H, int stride, int channels, int flags,
                   const orc_pt* diagram, const int32_t* edge_count, int scale,
                   uint8_t* out, int out_stride);
+/* ---- presentation in the viewer window (F4) ----
+   mode 0: the cells (VBO_RENDER), 1: one square per pixel (GL_QUADS);
+   view_flags bit 0: 4 samples per pixel, bit 1: window row 0 written first
+   is the top row.  diagram/edge_count unused (may be NULL) in mode 1. */
+int orc_present(const uint8_t* img, int W, int H, int stride, int channels, int flags,
+                const orc_pt* diagram, const int32_t* edge_count, int mode, int scale_x100,
+                int tx, int ty, int win_w, int win_h, int view_flags, uint8_t* out, int out_stride);
 int orc_depixelize(const uint8_t* img, int W, int H, int stride, int channels, int flags,
                    int scale, int subdivide, uint8_t* out, int out_stride,
                    uint8_t* graph_out, int32_t* edge_count_out, orc_pt* diagram_out,

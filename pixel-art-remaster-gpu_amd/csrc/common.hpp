@@ -14,6 +14,12 @@ int fail(int code, const char* fmt, ...) __attribute__((format(printf, 2, 3)));
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Flags the library knows; the stage and per-frame paths take no indexed frames.
+constexpr uint32_t kKnownFlags = DPX_RGB | DPX_TOPDOWN | DPX_ALPHA | DPX_FLIP_OUT | DPX_INDEXED;
+
+// Bytes of a DPX_INDEXED palette: 256 four-byte entries.
+constexpr size_t kPaletteBytes = 1024;
+
 // Returns DPX_ENODEV when the process sees no GPU.
 int require_device();
 

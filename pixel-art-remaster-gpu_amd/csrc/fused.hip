@@ -51,7 +51,7 @@ struct DevItem {
     int32_t out_stride, scale, subdivide;
     int32_t aligned16;  // output rows 16-byte aligned
     int32_t graph_w4;   // graph rows 4-byte aligned (W % 4 == 0)
-    int32_t pad[2];
+    uint64_t pal_off;   // DPX_INDEXED: the palette's byte offset in the input arena
 };
 
 // ---------------------------------------------------------------------------
@@ -199,17 +199,32 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
     // pass 1.  Byte offsets within an item fit 31 bits (checked on the host).
     const uint8_t* base = in + it.in_off;
     const bool rgb = it.flags & DPX_RGB, topdown = it.flags & DPX_TOPDOWN;
+    const bool indexed = it.flags & DPX_INDEXED;
     const bool alpha = (it.flags & DPX_ALPHA) && it.ch == 4;
     const uint32_t o0 = rgb ? 2 : 0, o2 = 2 - o0;
+    if (indexed) {
+        // the lane words of the 256 palette entries, in pass 2's rows until
+        // pass 2 writes them
+        const uint32_t e = reinterpret_cast<const uint32_t*>(in + it.pal_off)[t];
+        const int c0 = (e >> (8 * o0)) & 0xFF, c1 = (e >> 8) & 0xFF, c2 = (e >> (8 * o2)) & 0xFF;
+        uint32_t lw = yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque;
+        if ((it.flags & DPX_ALPHA) && (e >> 24) == 0) lw = kLaneTransparent;
+        s_h32[t] = lw;
+        __syncthreads();
+    }
     for (int k = t; k < GLX * GLY; k += 256) {
         int x = x0 - 1 + k % GLX, y = y0 - 2 + k / GLX;
         uint32_t lw = 0;
         if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
             int row = topdown ? it.H - 1 - y : y;
             uint32_t off = __umul24(row, it.stride) + __umul24(x, it.ch);  // operands < 2^24
-            int c0 = base[off + o0], c1 = base[off + 1], c2 = base[off + o2];
-            lw = yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque;
-            if (alpha && base[off + 3] == 0) lw = kLaneTransparent;
+            if (indexed) {
+                lw = s_h32[base[off]];
+            } else {
+                int c0 = base[off + o0], c1 = base[off + 1], c2 = base[off + o2];
+                lw = yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque;
+                if (alpha && base[off + 3] == 0) lw = kLaneTransparent;
+            }
         }
         s_lane[k] = lw;
     }
@@ -340,14 +355,22 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         const uint8_t* base = in + it.in_off;
         const uint32_t o0 = (it.flags & DPX_RGB) ? 2 : 0, o2 = 2 - o0;
         const bool topdown = it.flags & DPX_TOPDOWN, four = it.ch == 4;
+        const bool indexed = it.flags & DPX_INDEXED;
+        const uint32_t* pal = reinterpret_cast<const uint32_t*>(in + it.pal_off);
         for (int k = t; k < RCX * RCY; k += 256) {
             int x = x0 - 1 + k % RCX, y = y0 - 1 + k / RCX;
             if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
                 int row = topdown ? it.H - 1 - y : y;
                 uint32_t off = __umul24(row, it.stride) + __umul24(x, it.ch);  // operands < 2^24
-                uint32_t a = four ? base[off + 3] : 255u;
-                s_col[k] = (uint32_t)base[off + o2] | (uint32_t)base[off + 1] << 8 | (uint32_t)base[off + o0] << 16 |
-                           a << 24;  // colorPos (kernel.cu:98-101)
+                if (indexed) {
+                    const uint32_t e = pal[base[off]];
+                    s_col[k] = ((e >> (8 * o2)) & 0xFFu) | (e & 0xFF00u) | ((e >> (8 * o0)) & 0xFFu) << 16 |
+                               (e & 0xFF000000u);
+                } else {
+                    uint32_t a = four ? base[off + 3] : 255u;
+                    s_col[k] = (uint32_t)base[off + o2] | (uint32_t)base[off + 1] << 8 |
+                               (uint32_t)base[off + o0] << 16 | a << 24;  // colorPos (kernel.cu:98-101)
+                }
             }
         }
     }
@@ -439,7 +462,8 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         s_info[k] = info;
     }
     if constexpr (SUB) {
-        ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags };
+        ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags,
+                    (it.flags & DPX_INDEXED) ? in + it.pal_off : nullptr };
         for (int k = t; k < RHX * RHY; k += 256) {  // hull (lx, ly): cell (x0 - 2 + lx, y0 - 2 + ly)
             int lx = k % RHX, ly = k / RHX;
             int c = (ly - 2 + GHY) * RWB + lx + 6;
@@ -488,7 +512,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     // the other tiles' loads need in the caches.
     const int OH = it.H * S;
     auto store_row = [&](int gx, int Y, const uint32_t* col) {
-        int row = (it.flags & DPX_TOPDOWN) ? (OH - 1 - Y) : Y;
+        int row = out_topdown(it.flags) ? (OH - 1 - Y) : Y;
         uint8_t* dst = out + it.out_off + (size_t)row * it.out_stride + (size_t)gx * S * 4;
         if constexpr (S % 4 == 0) {
             if (it.aligned16) {
@@ -832,7 +856,13 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         const dpx_item& a = items[k];
         DPX_CHECK(a.width > 0 && a.height > 0 && a.width <= 32768 && a.height <= 32768, DPX_EINVAL,
                   "item %d: bad size %dx%d", k, a.width, a.height);
-        DPX_CHECK(a.channels == 3 || a.channels == 4, DPX_EINVAL, "item %d: channels %d", k, a.channels);
+        DPX_CHECK((a.flags & ~kKnownFlags) == 0, DPX_EINVAL, "item %d: unknown flags 0x%x", k,
+                  a.flags & ~kKnownFlags);
+        const bool indexed = a.flags & DPX_INDEXED;
+        DPX_CHECK(indexed ? a.channels == 1 : (a.channels == 3 || a.channels == 4), DPX_EINVAL,
+                  "item %d: channels %d%s", k, a.channels, indexed ? " (indexed frames have 1)" : "");
+        DPX_CHECK(!indexed || a.palette_offset % 4 == 0, DPX_EINVAL, "item %d: palette offset %llu not a multiple of 4",
+                  k, (unsigned long long)a.palette_offset);
         DPX_CHECK(a.stride >= a.width * a.channels, DPX_EINVAL, "item %d: stride %d", k, a.stride);
         DPX_CHECK((int64_t)a.stride * a.height < (int64_t)1 << 31, DPX_EINVAL, "item %d: input larger than 2 GiB", k);
         DPX_CHECK(a.scale >= 1 && a.scale <= kMaxScale, DPX_EINVAL, "item %d: scale %d outside 1..%d", k, a.scale,
@@ -850,6 +880,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         d.stride = a.stride;
         d.ch = a.channels;
         d.flags = a.flags;
+        d.pal_off = indexed ? a.palette_offset : 0;
         d.out_stride = a.out_stride;
         d.scale = a.scale;
         d.subdivide = a.subdivide ? 1 : 0;

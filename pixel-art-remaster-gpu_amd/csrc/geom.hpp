@@ -357,21 +357,24 @@ __host__ __device__ inline int ear_clip(const int* x, const int* y, int n, uint8
 // Coverage rule of the raster (replaces the GL rasteriser).  A sample on an
 // edge counts as inside iff it is inside after the symbolic shift (+e, +e^2):
 // for a counter-clockwise edge (dx,dy) that is dy < 0, or dy == 0 and dx > 0.
-// Coordinates are integers in units of 1/(128*scale) of a cell.
+// Coordinates are integers in units of 1/(128*scale) of a cell (int), or of
+// 1/(512*scale_x100) of a cell for the presentation stage (long long).
 // ---------------------------------------------------------------------------
-__host__ __device__ inline bool edge_in(int px, int py, int qx, int qy, int sx, int sy)
+template <class T>
+__host__ __device__ inline bool edge_in(T px, T py, T qx, T qy, T sx, T sy)
 {
-    int dx = qx - px, dy = qy - py;
-    int e = dx * (sy - py) - dy * (sx - px);
+    T dx = qx - px, dy = qy - py;
+    T e = dx * (sy - py) - dy * (sx - px);
     return e > 0 || (e == 0 && (dy < 0 || (dy == 0 && dx > 0)));
 }
 
 // triangle (a,b,c) covers sample s; degenerate triangles cover nothing
-__host__ __device__ inline bool tri_covers(int ax, int ay, int bx, int by, int cx, int cy, int sx, int sy)
+template <class T>
+__host__ __device__ inline bool tri_covers(T ax, T ay, T bx, T by, T cx, T cy, T sx, T sy)
 {
-    int o = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+    T o = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
     if (o == 0) return false;
-    if (o < 0) { int tx = bx, ty = by; bx = cx; by = cy; cx = tx; cy = ty; }
+    if (o < 0) { T tx = bx, ty = by; bx = cx; by = cy; cx = tx; cy = ty; }
     return edge_in(ax, ay, bx, by, sx, sy) && edge_in(bx, by, cx, cy, sx, sy) &&
            edge_in(cx, cy, ax, ay, sx, sy);
 }

@@ -8,17 +8,31 @@
 
 namespace dpx {
 
-// One frame as the kernels see it.
+// One frame as the kernels see it.  An indexed frame (DPX_INDEXED, ch 1)
+// carries its palette of 256 four-byte entries in `pal`; the kernels then see
+// the 4-channel frame of those entries.
 struct ImgView {
     const uint8_t* p;
     int W, H, stride, ch;
     uint32_t flags;
+    const uint8_t* pal;
 };
 
+// Output row 0 is the top row: the input's row order, reversed by DPX_FLIP_OUT.
+__host__ __device__ inline bool out_topdown(uint32_t flags)
+{
+    return ((flags & DPX_TOPDOWN) != 0) != ((flags & DPX_FLIP_OUT) != 0);
+}
+
+// Bytes per pixel colour: 4 for an indexed frame (its palette entries).
+__host__ __device__ inline int colour_channels(const ImgView& im) { return im.pal ? 4 : im.ch; }
+
+// The colour bytes of pixel (i, j): in the frame, or its palette entry.
 __host__ __device__ inline const uint8_t* px_ptr(const ImgView& im, int i, int j)
 {
     int row = (im.flags & DPX_TOPDOWN) ? (im.H - 1 - j) : j;
-    return im.p + (size_t)row * im.stride + (size_t)i * im.ch;
+    const uint8_t* q = im.p + (size_t)row * im.stride + (size_t)i * im.ch;
+    return im.pal ? im.pal + 4u * q[0] : q;
 }
 
 // The reference's byte triple (pixel[0], pixel[1], pixel[2] of a BGR buffer,
@@ -33,7 +47,7 @@ __host__ __device__ inline Px load_px(const ImgView& im, int i, int j)
     Px r;
     if (im.flags & DPX_RGB) { r.c0 = q[2]; r.c1 = q[1]; r.c2 = q[0]; }
     else                    { r.c0 = q[0]; r.c1 = q[1]; r.c2 = q[2]; }
-    r.a = im.ch == 4 ? q[3] : 255;
+    r.a = colour_channels(im) == 4 ? q[3] : 255;
     return r;
 }
 
@@ -175,7 +189,7 @@ __host__ __device__ inline TjView tj_view(const ImgView& im)
 {
     TjView v;
     v.im = im;
-    v.raw = (im.ch == 3) && !(im.flags & DPX_TOPDOWN);
+    v.raw = (im.ch == 3) && !im.pal && !(im.flags & DPX_TOPDOWN);
     v.ws = v.raw ? (long long)im.stride : 3LL * im.W;
     v.size = v.ws * im.H;
     return v;
@@ -192,7 +206,7 @@ __host__ __device__ inline bool tj_equal(const TjView& v, long long a, long long
     long long pa = a / 3, pb = b / 3;
     const uint8_t* qa = px_ptr(v.im, (int)(pa % v.im.W), (int)(pa / v.im.W));
     const uint8_t* qb = px_ptr(v.im, (int)(pb % v.im.W), (int)(pb / v.im.W));
-    for (int c = 0; c < v.im.ch; c++)
+    for (int c = 0, n = colour_channels(v.im); c < n; c++)
         if (qa[c] != qb[c]) return false;
     return true;
 }

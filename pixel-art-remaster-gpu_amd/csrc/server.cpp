@@ -17,6 +17,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <condition_variable>
@@ -34,6 +35,7 @@ namespace {
 using Clock = std::chrono::steady_clock;
 
 constexpr int kLanes = 4;
+using dpx::kPaletteBytes;
 
 size_t align256(size_t v) { return (v + 255) / 256 * 256; }
 
@@ -89,6 +91,8 @@ struct StreamInfo {
     int W, H, ch;
     dpx_params params;
     bool open;
+    bool has_palette = false;                 // DPX_INDEXED streams
+    std::array<uint8_t, kPaletteBytes> palette{};
 };
 
 struct Request {
@@ -102,7 +106,11 @@ struct Request {
     uint8_t* out = nullptr;
     int out_stride = 0;
     size_t in_bytes = 0, out_bytes = 0;
-    size_t in_off = 0, out_off = 0;  // arena ranges while admitted
+    // DPX_INDEXED: the stream's palette when the frame was submitted; it
+    // travels after the frame, at in_off + pal_rel, in the input range
+    std::unique_ptr<std::array<uint8_t, kPaletteBytes>> palette;
+    size_t pal_rel = 0, in_alloc = 0;  // arena bytes of the input range (frame + palette)
+    size_t in_off = 0, out_off = 0;    // arena ranges while admitted
     Clock::time_point t_submit;
     int status = 1;  // 1 pending, 2 admitted, else the final dpx_status
     bool inject = false;
@@ -188,6 +196,7 @@ int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
         in_direct.push_back(in_pinned_block(r->img, r->in_bytes));
         out_direct.push_back(in_pinned_block(r->out, (rows - 1) * r->out_stride + row));
         if (!in_direct.back()) hin += align256(r->in_bytes);
+        if (r->palette) hin += align256(kPaletteBytes);
         if (!out_direct.back()) hout += align256(r->out_bytes);
     }
     if (live.empty()) return DPX_OK;
@@ -205,6 +214,13 @@ int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
         DPX_HIP(hipMemcpyAsync(srv->d_arena + r->in_off, src, r->in_bytes, hipMemcpyHostToDevice, lane.stream));
         dpx_item it;
         memset(&it, 0, sizeof(it));
+        if (r->palette) {
+            memcpy(lane.h_in + ho, r->palette->data(), kPaletteBytes);
+            DPX_HIP(hipMemcpyAsync(srv->d_arena + r->in_off + r->pal_rel, lane.h_in + ho, kPaletteBytes,
+                                   hipMemcpyHostToDevice, lane.stream));
+            ho += align256(kPaletteBytes);
+            it.palette_offset = r->in_off + r->pal_rel;
+        }
         it.in_offset = r->in_off;
         it.out_offset = r->out_off;
         it.width = r->W;
@@ -256,9 +272,9 @@ void admit(dpx_server* srv, std::vector<Request*>& batch)
     while (!srv->pending.empty() && (int)batch.size() < srv->max_batch) {
         Request* r = srv->pending.front();
         size_t a, b;
-        if (!srv->arena.alloc(r->in_bytes, &a)) break;
+        if (!srv->arena.alloc(r->in_alloc, &a)) break;
         if (!srv->arena.alloc(r->out_bytes, &b)) {
-            srv->arena.release(a, r->in_bytes);
+            srv->arena.release(a, r->in_alloc);
             break;
         }
         r->in_off = a;
@@ -297,7 +313,7 @@ void lane_loop(dpx_server* srv, int li)
             srv->batches++;
             srv->batched_frames += batch.size();
             for (Request* r : batch) {
-                srv->arena.release(r->in_off, r->in_bytes);
+                srv->arena.release(r->in_off, r->in_alloc);
                 srv->arena.release(r->out_off, r->out_bytes);
                 r->status = r->inject ? DPX_EINJECTED : rc;
                 if (r->status == DPX_OK) {
@@ -385,7 +401,10 @@ extern "C" int dpx_server_open_stream(dpx_server* srv, int width, int height, in
     DPX_CHECK(srv && params && stream_id, DPX_EINVAL, "bad arguments");
     DPX_CHECK(width > 0 && height > 0 && width <= 32768 && height <= 32768, DPX_EINVAL, "bad size %dx%d", width,
               height);
-    DPX_CHECK(channels == 3 || channels == 4, DPX_EINVAL, "channels must be 3 or 4");
+    DPX_CHECK((params->flags & ~kKnownFlags) == 0, DPX_EINVAL, "unknown flags 0x%x", params->flags & ~kKnownFlags);
+    const bool indexed = params->flags & DPX_INDEXED;
+    DPX_CHECK(indexed ? channels == 1 : (channels == 3 || channels == 4), DPX_EINVAL,
+              indexed ? "indexed streams have 1 channel" : "channels must be 3 or 4");
     DPX_CHECK(params->scale >= 1 && params->scale <= 15, DPX_EINVAL, "scale %d outside 1..15", params->scale);
     DPX_CHECK(params->reserved == 0, DPX_EINVAL, "reserved must be 0");
     // size every lane for full batches of this stream's frames (outside the
@@ -396,7 +415,7 @@ extern "C" int dpx_server_open_stream(dpx_server* srv, int width, int height, in
         DPX_CHECK(!srv->closing, DPX_ECLOSED, "server closing");
     }
     // a batch holds at most as many frames as the HBM budget fits
-    const uint64_t per_frame = align256((size_t)width * channels * height) +
+    const uint64_t per_frame = align256((size_t)width * channels * height + (indexed ? kPaletteBytes + 3 : 0)) +
                                align256((size_t)pixels * params->scale * params->scale * 4);
     const int items = (int)std::max<uint64_t>(1, std::min<uint64_t>(srv->max_batch, srv->budget / per_frame));
     for (Lane& lane : srv->lanes)
@@ -407,8 +426,25 @@ extern "C" int dpx_server_open_stream(dpx_server* srv, int width, int height, in
     DPX_CHECK(!srv->closing, DPX_ECLOSED, "server closing");
     srv->reserved_pixels = std::max(pixels, srv->reserved_pixels.load());
     int id = srv->next_stream++;
-    srv->streams[id] = StreamInfo{ width, height, channels, *params, true };
+    StreamInfo& si = srv->streams[id];
+    si.W = width;
+    si.H = height;
+    si.ch = channels;
+    si.params = *params;
+    si.open = true;
     *stream_id = id;
+    return DPX_OK;
+}
+
+extern "C" int dpx_server_set_palette(dpx_server* srv, int stream_id, const uint8_t* palette)
+{
+    DPX_CHECK(srv && palette, DPX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> lock(srv->mu);
+    auto it = srv->streams.find(stream_id);
+    DPX_CHECK(it != srv->streams.end() && it->second.open, DPX_ECLOSED, "stream %d is not open", stream_id);
+    DPX_CHECK(it->second.params.flags & DPX_INDEXED, DPX_EINVAL, "stream %d is not indexed", stream_id);
+    memcpy(it->second.palette.data(), palette, kPaletteBytes);
+    it->second.has_palette = true;
     return DPX_OK;
 }
 
@@ -440,10 +476,17 @@ extern "C" int dpx_server_submit(dpx_server* srv, int stream_id, const uint8_t* 
     r->ch = si.ch;
     r->params = si.params;
     r->in_bytes = (size_t)stride * si.H;
+    r->in_alloc = r->in_bytes;
+    if (si.params.flags & DPX_INDEXED) {
+        DPX_CHECK(si.has_palette, DPX_EINVAL, "stream %d is indexed and has no palette yet", stream_id);
+        r->palette = std::make_unique<std::array<uint8_t, kPaletteBytes>>(si.palette);
+        r->pal_rel = (r->in_bytes + 3) / 4 * 4;
+        r->in_alloc = r->pal_rel + kPaletteBytes;
+    }
     r->out_bytes = (size_t)si.W * si.params.scale * 4 * si.H * si.params.scale;
-    if (align256(r->in_bytes) + align256(r->out_bytes) > srv->budget) {
+    if (align256(r->in_alloc) + align256(r->out_bytes) > srv->budget) {
         srv->rejected++;
-        return fail(DPX_EBUDGET, "frame needs %zu bytes of HBM, budget is %llu", r->in_bytes + r->out_bytes,
+        return fail(DPX_EBUDGET, "frame needs %zu bytes of HBM, budget is %llu", r->in_alloc + r->out_bytes,
                     (unsigned long long)srv->budget);
     }
     r->ticket = srv->next_ticket++;

@@ -28,13 +28,13 @@ __global__ void k_graph(ImgView im, uint8_t* __restrict__ graph)
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= im.W || j >= im.H) return;
-    Yuv c = yuv_of(load_px(im, i, j), im.flags, im.ch);
+    Yuv c = yuv_of(load_px(im, i, j), im.flags, colour_channels(im));
     uint32_t g = 0;
 #pragma unroll
     for (int e = 0; e < 8; e++) {
         int ni = i + edx(e), nj = j + edy(e);
         if (!in_frame(ni, nj, im.W, im.H)) continue;
-        if (similar(c, yuv_of(load_px(im, ni, nj), im.flags, im.ch))) g |= 1u << e;
+        if (similar(c, yuv_of(load_px(im, ni, nj), im.flags, colour_channels(im)))) g |= 1u << e;
     }
     graph[(size_t)j * im.W + i] = (uint8_t)g;
 }
@@ -187,7 +187,7 @@ __global__ void k_export_vbo(ImgView im, const dpx_point* __restrict__ diagram, 
     int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= im.W || j >= im.H) return;
     size_t idx = (size_t)j * im.W + i;
-    uint32_t rgba = cell_rgba(load_px(im, i, j), im.ch);
+    uint32_t rgba = cell_rgba(load_px(im, i, j), colour_channels(im));
     dpx_uchar4 c4;
     c4.x = (uint8_t)(rgba & 0xFF);
     c4.y = (uint8_t)((rgba >> 8) & 0xFF);
@@ -239,13 +239,97 @@ __global__ void k_raster_tri(ImgView im, const dpx_point* __restrict__ diagram, 
                     v[2 * q + 1] = (int)((p.y - fj) * 64.0f) * 2 * scale;
                 }
                 if (tri_covers(v[0], v[1], v[2], v[3], v[4], v[5], sx, sy)) {
-                    color = cell_rgba(load_px(im, ci, cj), im.ch);
+                    color = cell_rgba(load_px(im, ci, cj), colour_channels(im));
                     done = true;
                     break;
                 }
             }
         }
-    int row = (im.flags & DPX_TOPDOWN) ? (OH - 1 - Y) : Y;
+    int row = out_topdown(im.flags) ? (OH - 1 - Y) : Y;
+    *reinterpret_cast<uint32_t*>(out + (size_t)row * out_stride + (size_t)X * 4) = color;
+}
+
+// ---------------------------------------------------------------------------
+// Presentation: the GLUT window (display() callbacks.cpp:39-72, renderCuda
+// simpleVBO.cpp:185-440).  Window point (X + ox/8, Y + oy/8) maps to image
+// point ((8 (X - tx) + ox) * 100) / (8 * sc), sc the scale in hundredths.  In
+// units of 1/(512 sc) of a cell that is (8 (X - tx) + ox) * 6400 and a
+// 1/64-lattice vertex v is 64 v * 8 sc: both integers, so coverage is exact.
+// ---------------------------------------------------------------------------
+struct ViewGeom {
+    int mode, sc, tx, ty, W, H, aa, topdown;
+};
+
+__device__ inline long long floor_div(long long a, long long b)  // b > 0
+{
+    long long q = a / b;
+    return (a % b != 0 && a < 0) ? q - 1 : q;
+}
+
+// colour of the sample at window position (8 X + ox, 8 Y + oy) / 8
+__device__ inline uint32_t present_sample(const ImgView& im, const dpx_point* __restrict__ diagram,
+                                          const int32_t* __restrict__ edge_count, const ViewGeom& v, int X, int Y,
+                                          int ox, int oy)
+{
+    const long long unit = 512LL * v.sc;  // one cell
+    const long long px = (8LL * (X - v.tx) + ox) * 6400, py = (8LL * (Y - v.ty) + oy) * 6400;
+    const long long ci = floor_div(px, unit), cj = floor_div(py, unit);
+    const int cc = colour_channels(im);
+    if (v.mode == DPX_VIEW_QUADS) {
+        // the two triangles of pixel (ci, cj) cover [ci, ci+1) x [cj, cj+1)
+        // under the coverage rule (left and bottom edges in, right and top out)
+        if (ci < 0 || cj < 0 || ci >= im.W || cj >= im.H) return 0xFF000000u;
+        return cell_rgba(load_px(im, (int)ci, (int)cj), cc);
+    }
+    for (int dy = 1; dy >= -1; dy--)
+        for (int dx = 1; dx >= -1; dx--) {  // cells in reverse draw order: the first hit wins
+            long long i = ci + dx, j = cj + dy;
+            if (i < 0 || j < 0 || i >= im.W || j >= im.H) continue;
+            size_t k = (size_t)j * im.W + (size_t)i;
+            int nt = edge_count[k] - 2;
+            if (nt > kCellSize / 3) nt = kCellSize / 3;
+            const long long sx = px - i * unit, sy = py - j * unit;
+            const dpx_point* c = &diagram[k * kCellSize];
+            const float fi = (float)i, fj = (float)j;
+            for (int t = 0; t < nt; t++) {
+                long long w[6];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    dpx_point p = c[3 * t + q];
+                    w[2 * q] = (long long)(int)((p.x - fi) * 64.0f) * 8 * v.sc;
+                    w[2 * q + 1] = (long long)(int)((p.y - fj) * 64.0f) * 8 * v.sc;
+                }
+                if (tri_covers(w[0], w[1], w[2], w[3], w[4], w[5], sx, sy))
+                    return cell_rgba(load_px(im, (int)i, (int)j), cc);
+            }
+        }
+    return 0xFF000000u;  // clear colour (main.cpp:260)
+}
+
+__global__ void k_present(ImgView im, const dpx_point* __restrict__ diagram, const int32_t* __restrict__ edge_count,
+                          ViewGeom v, uint8_t* __restrict__ out, int out_stride)
+{
+    int X = blockIdx.x * blockDim.x + threadIdx.x;
+    int Y = blockIdx.y * blockDim.y + threadIdx.y;
+    if (X >= v.W || Y >= v.H) return;
+    uint32_t color;
+    if (!v.aa) {
+        color = present_sample(im, diagram, edge_count, v, X, Y, 4, 4);
+    } else {
+        // rotated 4x grid, averaged per channel with halves rounded up
+        const int sox[4] = { 3, 7, 1, 5 }, soy[4] = { 1, 3, 5, 7 };
+        uint32_t sum[4] = { 0, 0, 0, 0 };
+#pragma unroll
+        for (int s = 0; s < 4; s++) {
+            uint32_t c = present_sample(im, diagram, edge_count, v, X, Y, sox[s], soy[s]);
+#pragma unroll
+            for (int b = 0; b < 4; b++) sum[b] += (c >> (8 * b)) & 0xFFu;
+        }
+        color = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) color |= ((sum[b] + 2) >> 2) << (8 * b);
+    }
+    int row = v.topdown ? (v.H - 1 - Y) : Y;
     *reinterpret_cast<uint32_t*>(out + (size_t)row * out_stride + (size_t)X * 4) = color;
 }
 
@@ -388,10 +472,12 @@ __global__ __launch_bounds__(64) void k_border_chains(const uint8_t* __restrict_
     }
 }
 
-int check_img(const void* p, int W, int H, int stride, int ch)
+int check_img(const void* p, int W, int H, int stride, int ch, uint32_t flags)
 {
     DPX_CHECK(p != nullptr, DPX_EINVAL, "null image pointer");
     DPX_CHECK(W > 0 && H > 0 && W <= 32768 && H <= 32768, DPX_EINVAL, "bad frame size %dx%d", W, H);
+    DPX_CHECK((flags & ~kKnownFlags) == 0, DPX_EINVAL, "unknown flags 0x%x", flags & ~kKnownFlags);
+    DPX_CHECK(!(flags & DPX_INDEXED), DPX_EINVAL, "indexed frames: batched and serving paths only");
     DPX_CHECK(ch == 3 || ch == 4, DPX_EINVAL, "channels must be 3 or 4, got %d", ch);
     DPX_CHECK(stride >= W * ch, DPX_EINVAL, "stride %d < width*channels %d", stride, W * ch);
     return DPX_OK;
@@ -405,7 +491,7 @@ int check_img(const void* p, int W, int H, int stride, int ch)
 extern "C" int dpx_stage_similarity_graph(const uint8_t* d_img, int W, int H, int stride, int ch, uint32_t flags,
                                           uint8_t* d_graph, void* stream)
 {
-    if (int rc = check_img(d_img, W, H, stride, ch)) return rc;
+    if (int rc = check_img(d_img, W, H, stride, ch, flags)) return rc;
     DPX_CHECK(d_graph, DPX_EINVAL, "null graph pointer");
     ImgView im{ d_img, W, H, stride, ch, flags };
     hipLaunchKernelGGL(k_graph, grid2(W, H), dim3(kB, kB), 0, as_stream(stream), im, d_graph);
@@ -444,7 +530,7 @@ extern "C" int dpx_stage_subdivide(const uint8_t* d_img, int W, int H, int strid
                                    const uint8_t* d_graph, dpx_point* d_diagram, dpx_point* d_diagram_aux,
                                    int32_t* d_edge_count, int32_t* d_edge_count_aux, void* stream)
 {
-    if (int rc = check_img(d_img, W, H, stride, ch)) return rc;
+    if (int rc = check_img(d_img, W, H, stride, ch, flags)) return rc;
     DPX_CHECK(d_graph && d_diagram && d_diagram_aux && d_edge_count && d_edge_count_aux, DPX_EINVAL,
               "bad arguments");
     size_t N = (size_t)W * H;
@@ -470,7 +556,7 @@ extern "C" int dpx_stage_export_vbo(const uint8_t* d_img, int W, int H, int stri
                                     const dpx_point* d_diagram, const int32_t* d_edge_count, dpx_point* d_pos,
                                     dpx_uchar4* d_color, void* stream)
 {
-    if (int rc = check_img(d_img, W, H, stride, ch)) return rc;
+    if (int rc = check_img(d_img, W, H, stride, ch, flags)) return rc;
     DPX_CHECK(d_diagram && d_edge_count && (d_pos || d_color), DPX_EINVAL, "bad arguments");
     ImgView im{ d_img, W, H, stride, ch, flags };
     hipLaunchKernelGGL(k_export_vbo, grid2(W, H), dim3(kB, kB), 0, as_stream(stream), im, d_diagram, d_edge_count,
@@ -483,7 +569,7 @@ extern "C" int dpx_stage_rasterize(const uint8_t* d_img, int W, int H, int strid
                                    const dpx_point* d_diagram, const int32_t* d_edge_count, int scale,
                                    uint8_t* d_out, int out_stride, void* stream)
 {
-    if (int rc = check_img(d_img, W, H, stride, ch)) return rc;
+    if (int rc = check_img(d_img, W, H, stride, ch, flags)) return rc;
     DPX_CHECK(d_diagram && d_edge_count && d_out, DPX_EINVAL, "bad arguments");
     DPX_CHECK(scale >= 1 && scale <= 32, DPX_EINVAL, "scale %d outside 1..32", scale);
     DPX_CHECK((long long)W * scale <= 65536 && (long long)H * scale <= 65536, DPX_EINVAL, "output too large");
@@ -491,6 +577,35 @@ extern "C" int dpx_stage_rasterize(const uint8_t* d_img, int W, int H, int strid
     ImgView im{ d_img, W, H, stride, ch, flags };
     hipLaunchKernelGGL(k_raster_tri, grid2(W * scale, H * scale), dim3(kB, kB), 0, as_stream(stream), im, d_diagram,
                        d_edge_count, scale, d_out, out_stride);
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+
+extern "C" int dpx_stage_present(const uint8_t* d_img, int W, int H, int stride, int ch, uint32_t flags,
+                                 const dpx_point* d_diagram, const int32_t* d_edge_count, const dpx_view* view,
+                                 uint8_t* d_out, int out_stride, void* stream)
+{
+    if (int rc = check_img(d_img, W, H, stride, ch, flags)) return rc;
+    DPX_CHECK(view && d_out, DPX_EINVAL, "bad arguments");
+    DPX_CHECK(view->mode == DPX_VIEW_CELLS || view->mode == DPX_VIEW_QUADS, DPX_EINVAL, "unknown view mode %d",
+              view->mode);
+    DPX_CHECK(view->mode == DPX_VIEW_QUADS || (d_diagram && d_edge_count), DPX_EINVAL,
+              "the cells view needs the triangulated diagram");
+    DPX_CHECK(view->scale_x100 >= 25 && view->scale_x100 <= 3200, DPX_EINVAL, "scale_x100 %d outside 25..3200",
+              view->scale_x100);
+    DPX_CHECK(view->translate_x >= -65536 && view->translate_x <= 65536 && view->translate_y >= -65536 &&
+                  view->translate_y <= 65536,
+              DPX_EINVAL, "translation outside -65536..65536");
+    DPX_CHECK(view->width >= 1 && view->width <= 16384 && view->height >= 1 && view->height <= 16384, DPX_EINVAL,
+              "window %dx%d outside 1..16384", view->width, view->height);
+    DPX_CHECK((view->flags & ~(DPX_VIEW_AA | DPX_VIEW_TOPDOWN)) == 0 && view->reserved == 0, DPX_EINVAL,
+              "unknown view flags");
+    DPX_CHECK(out_stride >= 4 * view->width && out_stride % 4 == 0, DPX_EINVAL, "bad out_stride %d", out_stride);
+    ImgView im{ d_img, W, H, stride, ch, flags };
+    ViewGeom v{ view->mode, view->scale_x100, view->translate_x, view->translate_y, view->width, view->height,
+                (view->flags & DPX_VIEW_AA) ? 1 : 0, (view->flags & DPX_VIEW_TOPDOWN) ? 1 : 0 };
+    hipLaunchKernelGGL(k_present, grid2(v.W, v.H), dim3(kB, kB), 0, as_stream(stream), im, d_diagram, d_edge_count, v,
+                       d_out, out_stride);
     DPX_HIP(hipGetLastError());
     return DPX_OK;
 }
@@ -588,8 +703,8 @@ extern "C" int dpx_frame(const uint8_t* img, int W, int H, int stride, int ch, c
                          dpx_point* diagram_out)
 {
     using namespace dpx;
-    if (int rc = check_img(img, W, H, stride, ch)) return rc;
     DPX_CHECK(params, DPX_EINVAL, "null params");
+    if (int rc = check_img(img, W, H, stride, ch, params->flags)) return rc;
     DPX_CHECK(params->reserved == 0, DPX_EINVAL, "reserved must be 0");
     if (int rc = require_device()) return rc;
     int scale = params->scale;
@@ -625,7 +740,7 @@ extern "C" dpx_point* launch_kernel(dpx_point* pos, dpx_uchar4* colorPos, float 
     using namespace dpx;
     (void)time;
     int W = img_width, H = img_height;
-    if (check_img(img_data, W, H, img_widthstep, 3) || require_device()) return nullptr;
+    if (check_img(img_data, W, H, img_widthstep, 3, 0) || require_device()) return nullptr;
     size_t N = (size_t)W * H;
     FrameBuffers fb;
     if (alloc_frame(fb, W, H, img_widthstep, subdivide)) return nullptr;

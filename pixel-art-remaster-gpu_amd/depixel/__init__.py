@@ -18,7 +18,9 @@ LIB_PATH = os.path.join(os.path.dirname(_HERE), "build", "libdepixel.so")
 HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "depixel.h")
 
 CELL_SIZE = 45
-RGB, TOPDOWN, ALPHA = 1, 2, 4
+RGB, TOPDOWN, ALPHA, FLIP_OUT, INDEXED = 1, 2, 4, 8, 16
+VIEW_CELLS, VIEW_QUADS = 0, 1
+VIEW_AA, VIEW_TOPDOWN = 1, 2
 
 OK, EINVAL, ENOMEM, EHIP, ENODEV, EBUDGET, ETIMEOUT, ECANCELLED, ECLOSED, EINJECTED = (
     0, -1, -2, -3, -4, -5, -6, -7, -8, -9)
@@ -35,7 +37,13 @@ class Params(ctypes.Structure):
 class Item(ctypes.Structure):
     _fields_ = [("in_offset", c_uint64), ("out_offset", c_uint64), ("width", c_int), ("height", c_int),
                 ("stride", c_int), ("channels", c_int), ("scale", c_int), ("subdivide", c_int),
-                ("flags", c_uint32), ("out_stride", c_int)]
+                ("flags", c_uint32), ("out_stride", c_int), ("palette_offset", c_uint64)]
+
+
+class View(ctypes.Structure):
+    """dpx_view: the GLUT window a frame is presented in (defaults: the reference's)."""
+    _fields_ = [("mode", c_int), ("scale_x100", c_int), ("translate_x", c_int), ("translate_y", c_int),
+                ("width", c_int), ("height", c_int), ("flags", c_uint32), ("reserved", c_int)]
 
 
 class BatchTimes(ctypes.Structure):
@@ -80,6 +88,8 @@ _SIGS = {
                                      c_void_p, c_void_p]),
     "dpx_stage_rasterize": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_uint32, c_void_p, c_void_p, c_int,
                                     c_void_p, c_int, c_void_p]),
+    "dpx_stage_present": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_uint32, c_void_p, c_void_p, POINTER(View),
+                                  c_void_p, c_int, c_void_p]),
     "dpx_stage_components": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p]),
     "dpx_stage_border_chains": (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int,
                                         c_void_p, c_void_p]),
@@ -93,6 +103,7 @@ _SIGS = {
     "dpx_server_destroy": (None, [c_void_p]),
     "dpx_server_open_stream": (c_int, [c_void_p, c_int, c_int, c_int, POINTER(Params), POINTER(c_int)]),
     "dpx_server_close_stream": (c_int, [c_void_p, c_int]),
+    "dpx_server_set_palette": (c_int, [c_void_p, c_int, c_void_p]),
     "dpx_server_submit": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(c_uint64)]),
     "dpx_server_wait": (c_int, [c_void_p, c_uint64, c_int]),
     "dpx_server_cancel": (c_int, [c_void_p, c_uint64]),
@@ -279,6 +290,12 @@ class Server:
 
     def close_stream(self, sid: int):
         check(lib.dpx_server_close_stream(self.h, sid))
+
+    def set_palette(self, sid: int, palette: np.ndarray):
+        """The palette (256 x 4 bytes) of an INDEXED stream, for frames submitted from now on."""
+        pal = np.ascontiguousarray(palette, dtype=np.uint8).reshape(-1)
+        assert pal.size == 1024
+        check(lib.dpx_server_set_palette(self.h, sid, _ptr(pal)))
 
     def submit(self, sid: int, img: np.ndarray, out: np.ndarray) -> int:
         """Queues a host frame; `out` (rows of >= 4*scale*W bytes) is filled on completion."""

@@ -55,6 +55,25 @@ def test_argument_checks_need_no_gpu(D):
     assert D.lib.dpx_frame(img.ctypes.data, 4, 4, 12, 2, ctypes.byref(p), None, 0, None, None, None) == D.EINVAL
     assert D.lib.dpx_frame(img.ctypes.data, 4, 4, 8, 3, ctypes.byref(p), None, 0, None, None, None) == D.EINVAL
     assert "stride" in D.last_error()
+    # unknown flags, and indexed frames outside the batched and serving paths
+    for flags, msg in ((64, "unknown flags"), (D.INDEXED, "indexed")):
+        q = D.Params(4, 0, flags, 0)
+        assert D.lib.dpx_frame(img.ctypes.data, 4, 4, 12, 3, ctypes.byref(q), None, 0, None, None, None) == D.EINVAL
+        assert msg in D.last_error()
+        assert D.lib.dpx_stage_similarity_graph(img.ctypes.data, 4, 4, 12, 3, flags, img.ctypes.data, None) == D.EINVAL
+    # views outside the documented ranges
+    one = ctypes.c_void_p(img.ctypes.data)
+    for view, msg in ((D.View(2, 800, 1, 1, 64, 64, 0, 0), "view mode"),
+                      (D.View(0, 800, 1, 1, 64, 64, 0, 0), "triangulated diagram"),
+                      (D.View(1, 24, 1, 1, 64, 64, 0, 0), "scale_x100"),
+                      (D.View(1, 800, 1, 1, 0, 64, 0, 0), "window"),
+                      (D.View(1, 800, 1, 1, 64, 64, 4, 0), "view flags"),
+                      (D.View(1, 800, 70000, 1, 64, 64, 0, 0), "translation")):
+        rc = D.lib.dpx_stage_present(one, 4, 4, 12, 3, 0, None, None, ctypes.byref(view), one, 256, None)
+        assert rc == D.EINVAL and msg in D.last_error(), (msg, D.last_error())
+    ok = D.View(1, 800, 1, 1, 64, 64, 0, 0)
+    assert D.lib.dpx_stage_present(one, 4, 4, 12, 3, 0, None, None, ctypes.byref(ok), one, 252, None) == D.EINVAL
+    assert "out_stride" in D.last_error()
 
 
 @pytest.mark.skipif(os.environ.get("HIP_VISIBLE_DEVICES") is None and os.path.exists("/dev/kfd"),

@@ -316,6 +316,13 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     __shared__ int8_t s_hull[SUB ? RHX * RHY * 17 : 1];
     __shared__ uint16_t s_list[RTX * RTY];  // own cells that need the candidate search
     __shared__ int s_nlist;
+    // resolved pixels of the first PX_CELLS listed cells, so that every own
+    // cell's rows are stored together (a line is then written by one store
+    // instruction instead of being split between the direct and searched cells)
+    constexpr int PX_WORDS = 2048, PX_CELLS = PX_WORDS / (S * S);
+    static_assert(PX_CELLS >= 1, "at least one buffered cell");
+    __shared__ uint32_t s_px[PX_WORDS];
+    __shared__ uint16_t s_idx[RTX * RTY];  // list index of a listed own cell
     const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
 
     const int t = threadIdx.x;
@@ -540,14 +547,17 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
 
     // Own cells, one per thread: a cell whose S*S pixels all show its own
     // colour is written directly, the others are listed for the candidate
-    // search.  The list is complete before any pixel is written, so the
-    // searches' table reads are in flight while the direct writes issue.
+    // search, which resolves their pixels into LDS first.
     const int cx = t % RTX, cy = t / RTX;
     const int kown = (cy + 1) * RCX + cx + 1;
     const bool mine = x0 + cx < it.W && y0 + cy < it.H;
     const bool simple = mine && simple_cell(s_info[kown], s_info[kown + RCX - 1], s_info[kown + RCX],
                                             s_info[kown + RCX + 1], s_info[kown + 1]);
-    if (mine && !simple) s_list[atomicAdd(&s_nlist, 1)] = (uint16_t)t;
+    if (mine && !simple) {
+        const int i = atomicAdd(&s_nlist, 1);
+        s_list[i] = (uint16_t)t;
+        s_idx[t] = (uint16_t)i;
+    }
     __syncthreads();
 
     // One unit = one output row (S pixels) of a listed cell, resolved
@@ -596,11 +606,22 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 }
             }
         }
-        store_row(x0 + ux, (y0 + uy) * S + ky, col);
+        if (u / S < PX_CELLS) {
+#pragma unroll
+            for (int q = 0; q < S; q++) s_px[u * S + q] = col[q];
+        } else {
+            store_row(x0 + ux, (y0 + uy) * S + ky, col);  // beyond the buffer: written directly
+        }
     };
 
-    uint32_t rows0[9];
-    if (t < nunits) fetch_rows(t, rows0);
+    if (nunits > 0) {  // uniform: read from LDS after the barrier
+        for (int u = t; u < nunits; u += 256) {
+            uint32_t rows[9];
+            fetch_rows(u, rows);
+            resolve_unit(u, rows);
+        }
+        __syncthreads();
+    }
     if (simple) {
         uint32_t col[S];
         uint32_t c = s_col[kown];
@@ -608,12 +629,17 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         for (int q = 0; q < S; q++) col[q] = c;
 #pragma unroll
         for (int ky = 0; ky < S; ky++) store_row(x0 + cx, (y0 + cy) * S + ky, col);
-    }
-    if (t < nunits) resolve_unit(t, rows0);
-    for (int u = t + 256; u < nunits; u += 256) {
-        uint32_t rows[9];
-        fetch_rows(u, rows);
-        resolve_unit(u, rows);
+    } else if (mine) {
+        const int i = s_idx[t];
+        if (i < PX_CELLS) {
+#pragma unroll
+            for (int ky = 0; ky < S; ky++) {
+                uint32_t col[S];
+#pragma unroll
+                for (int q = 0; q < S; q++) col[q] = s_px[(i * S + ky) * S + q];
+                store_row(x0 + cx, (y0 + cy) * S + ky, col);
+            }
+        }
     }
 }
 

@@ -156,11 +156,33 @@ struct TileAt {
     int item, tx, ty;
 };
 
+// With RUN > 1 the blocks are permuted in groups of 8 * RUN: the hardware
+// hands consecutive blocks to the 8 XCDs in turn, and within a group XCD x
+// is given the RUN consecutive tiles x * RUN .. x * RUN + RUN - 1 (a row of
+// horizontally adjacent tiles), so the input and graph lines that adjacent
+// tiles share are fetched into one XCD's L2 once, not into several.  Blocks
+// past the last whole group keep their place.
+template <int RUN>
 __device__ inline TileAt locate_tile(const uint2* __restrict__ tile_map, int grid3d)
 {
-    if (grid3d) return TileAt{ (int)blockIdx.z, (int)blockIdx.x, (int)blockIdx.y };
-    const uint2 m = tile_map[blockIdx.x];
-    return TileAt{ (int)m.x, (int)(m.y & 0xFFFFu), (int)(m.y >> 16) };
+    if constexpr (RUN == 1) {
+        if (grid3d) return TileAt{ (int)blockIdx.z, (int)blockIdx.x, (int)blockIdx.y };
+        const uint2 m = tile_map[blockIdx.x];
+        return TileAt{ (int)m.x, (int)(m.y & 0xFFFFu), (int)(m.y >> 16) };
+    } else {
+        const uint32_t n = gridDim.x * gridDim.y * gridDim.z;
+        uint32_t b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+        if (b < n / (8 * RUN) * (8 * RUN)) {
+            const uint32_t w = b % (8 * RUN);
+            b += (w % 8) * RUN + w / 8 - w;
+        }
+        if (grid3d) {
+            const uint32_t plane = gridDim.x * gridDim.y, r = b % plane;
+            return TileAt{ (int)(b / plane), (int)(r % gridDim.x), (int)(r / gridDim.x) };
+        }
+        const uint2 m = tile_map[b];
+        return TileAt{ (int)m.x, (int)(m.y & 0xFFFFu), (int)(m.y >> 16) };
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -192,7 +214,7 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
     __shared__ uint32_t s_h32[GHY * GHP / 4];
     uint8_t* s_h = reinterpret_cast<uint8_t*>(s_h32);
     const int t = threadIdx.x;
-    const TileAt at = locate_tile(tile_map, grid3d);
+    const TileAt at = locate_tile<1>(tile_map, grid3d);
     const DevItem it = items[at.item];
     const int x0 = at.tx * GTX, y0 = at.ty * GTY;
 
@@ -326,7 +348,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
 
     const int t = threadIdx.x;
-    const TileAt at = locate_tile(tile_map, grid3d);
+    const TileAt at = locate_tile<8>(tile_map, grid3d);
     const DevItem it = items[at.item];
     const int x0 = at.tx * RTX, y0 = at.ty * RTY;
     const uint8_t* aux = gaux + it.g_off;

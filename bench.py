@@ -203,6 +203,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference for HBM byte counters)")
+    ap.add_argument("--subdivide", action="store_true",
+                    help="c0 with corner-cutting subdivision on (the reference's 's' key)")
     ap.add_argument("--config", default="c0", choices=["c0", "c1", "c2", "c3", "c4"],
                     help="c0 (default): the headline 256x224 -> 4x line; c1 4096 alpha sprites -> 6x; "
                          "c2 64 frames of five console sizes -> 4x; c3 128 served 320x224 streams at 60 Hz; "
@@ -230,7 +232,8 @@ def main():
     host = np.stack([D.synth_frame(0, seed, W, H, 3, stride) for seed in range(max_b)])
     d_in = torch.from_numpy(host.reshape(-1)).to(dev)
     d_out = torch.empty(max_b * OUT_BYTES, dtype=torch.uint8, device=dev)
-    items_all = [D.Item(k * IN_BYTES, k * OUT_BYTES, W, H, stride, 3, S, 0, 0, W * S * 4) for k in range(max_b)]
+    sub = 1 if args.subdivide else 0
+    items_all = [D.Item(k * IN_BYTES, k * OUT_BYTES, W, H, stride, 3, S, sub, 0, W * S * 4) for k in range(max_b)]
     ctx = D.Context(0)
 
     def run_steps(B, steps, warmup, record=True):
@@ -281,16 +284,16 @@ def main():
     # parity spot check of the timed output against the per-frame path
     k = best_b - 1
     got = d_out[k * OUT_BYTES:(k + 1) * OUT_BYTES].cpu().numpy()
-    rgba, _, _ = D.frame(host[k], W, H, 3, scale=S)
+    rgba, _, _ = D.frame(host[k], W, H, 3, scale=S, subdivide=bool(sub))
     parity_ok = bool(np.array_equal(got, rgba.reshape(-1)))
 
     # plain per-frame path: one dpx_frame call per frame (host in/out, per-call allocation)
     plain_fps = None
     if args.plain_frames > 0:
-        D.frame(host[0], W, H, 3, scale=S)
+        D.frame(host[0], W, H, 3, scale=S, subdivide=bool(sub))
         t0 = time.perf_counter()
         for f in range(args.plain_frames):
-            D.frame(host[f % max_b], W, H, 3, scale=S)
+            D.frame(host[f % max_b], W, H, 3, scale=S, subdivide=bool(sub))
         plain_fps = args.plain_frames / (time.perf_counter() - t0)
 
     if args.calibrate_copy:
@@ -306,7 +309,7 @@ def main():
     if os.path.exists(tp):
         with open(tp) as f:
             tj = json.load(f)
-        if tj.get("batch") == best_b:
+        if tj.get("batch") == best_b and not sub:
             traffic = tj.get("bytes_per_launch")
 
     cpu = None
@@ -323,7 +326,7 @@ def main():
         def worker(c):
             k = c
             while time.perf_counter() - t0 < args.cpu_seconds or counts[c] == 0:
-                O.depixelize(host[k % max_b], W, H, scale=S)
+                O.depixelize(host[k % max_b], W, H, scale=S, subdivide=bool(sub))
                 counts[c] += 1
                 k += cores
 
@@ -353,8 +356,8 @@ def main():
         "dtype": "u8",
         "data": "synthetic",
         "config": {"workload": f"batched 256x224 -> 4x (1024x896 RGBA8) frames, 16-colour synthetic pixel art, "
-                               f"B={best_b} frames per step, subdivision off", "global_batch": best_b,
-                   "frame": "256x224", "scale": S},
+                               f"B={best_b} frames per step, subdivision {'on' if sub else 'off'}",
+                   "global_batch": best_b, "frame": "256x224", "scale": S},
         "p50_ms": round(float(np.percentile(lat, 50)), 4),
         "p99_ms": round(float(np.percentile(lat, 99)), 4),
         "plain_frames_per_s": round(plain_fps, 2) if plain_fps else None,
@@ -364,7 +367,7 @@ def main():
         "sweep": sweep,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "kernel": "kf_raster<4,false>",
+                     "kernel": f"kf_raster<4,{'true' if sub else 'false'}>",
                      "algorithmic_bytes_per_frame": ALGO_BYTES},
         "cpu_baseline": cpu,
     }

@@ -19,6 +19,12 @@
 // (kmin+b, kmin+r) of the cell's local sample grid is covered, kmin =
 // -(S/2)-1, MW = 2S+2 rows and columns (local [-0.5, 1.5] of the cell); and
 // into a 16-bit summary per key (kt_info, coverage_info).
+//
+// This file is compiled twice.  Built with DPX_RASTER_SUB_TU it yields only
+// the raster kernels with subdivision (pick_raster_sub): their per-thread
+// polygon arrays must stay in memory rather than be promoted to registers,
+// which takes a compiler limit that would cost the other kernels.  Built
+// without it, it yields everything else.
 #include <stdlib.h>
 #include <string.h>
 
@@ -37,8 +43,8 @@ namespace dpx {
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
 
-constexpr int kMaxScale = 15;
-constexpr int kKeys = 4096;
+[[maybe_unused]] constexpr int kMaxScale = 15;
+[[maybe_unused]] constexpr int kKeys = 4096;
 constexpr int kHullBytes = 32;  // n, x[8], y[8] (quarter units), padding
 
 __host__ __device__ constexpr int mask_kmin(int S) { return -(S / 2) - 1; }
@@ -57,6 +63,7 @@ struct DevItem {
 // ---------------------------------------------------------------------------
 // tables
 // ---------------------------------------------------------------------------
+#ifndef DPX_RASTER_SUB_TU
 __global__ void kt_hull(int8_t* __restrict__ table)
 {
     int key = blockIdx.x * blockDim.x + threadIdx.x;
@@ -70,6 +77,7 @@ __global__ void kt_hull(int8_t* __restrict__ table)
         e[9 + t] = (int8_t)(t < n ? hy[t] : 0);
     }
 }
+#endif
 
 // Coverage bits of one sample row of a polygon given in 1/64 units and
 // already ear-clipped (tri, nt).  Units of the test: 1/(128 S) of a cell.
@@ -92,6 +100,7 @@ __device__ inline uint32_t coverage_row(const int* x, const int* y, const uint8_
     return bits;
 }
 
+#ifndef DPX_RASTER_SUB_TU
 __global__ void kt_mask(const int8_t* __restrict__ hull_table, int S, uint32_t* __restrict__ mask)
 {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -107,6 +116,7 @@ __global__ void kt_mask(const int8_t* __restrict__ hull_table, int S, uint32_t* 
     int nt = ear_clip(x, y, n, tri, &ok);
     mask[gid] = coverage_row(x, y, tri, nt, S, r);
 }
+#endif
 
 // Summary of a cell's coverage rows at scale S: bit 0 = it covers every
 // sample of its own square; bit 1 + (dy+1)*3 + (dx+1) = it covers a sample of
@@ -130,12 +140,14 @@ __host__ __device__ inline uint32_t coverage_info(const uint32_t* rows, int S)
     return info | (full ? 1u : 0u);
 }
 
+#ifndef DPX_RASTER_SUB_TU
 __global__ void kt_info(const uint32_t* __restrict__ mask, int S, uint16_t* __restrict__ info)
 {
     int key = blockIdx.x * blockDim.x + threadIdx.x;
     if (key >= kKeys) return;
     info[key] = (uint16_t)coverage_info(mask + (size_t)key * mask_w(S), S);
 }
+#endif
 
 // A cell whose own square it covers fully, and into which none of the four
 // cells drawn after it (up-left, up, up-right, right) reaches, shows its own
@@ -185,6 +197,7 @@ __device__ inline TileAt locate_tile(const uint2* __restrict__ tile_map, int gri
     }
 }
 
+#ifndef DPX_RASTER_SUB_TU
 // ---------------------------------------------------------------------------
 // kf_graph: similarity graph + trivial crossing removal
 // ---------------------------------------------------------------------------
@@ -289,6 +302,7 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         }
     }
 }
+#endif  // DPX_RASTER_SUB_TU
 
 // ---------------------------------------------------------------------------
 // kf_raster: heuristic crossing resolution, cells (+ subdivision) and the
@@ -668,35 +682,41 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
 using RasterFn = void (*)(const DevItem*, const uint2*, int, const uint8_t*, const uint8_t*, uint8_t*,
                           const int8_t*, const uint32_t*, const uint16_t*, uint8_t*);
 
-template <int S>
-RasterFn raster_fn(bool sub)
-{
-    return sub ? (RasterFn)kf_raster<S, true> : (RasterFn)kf_raster<S, false>;
-}
-
-RasterFn pick_raster(int S, bool sub)
+// The raster kernel of scale S (1..15) with (SUB) or without subdivision.
+template <bool SUB>
+RasterFn pick_raster_of(int S)
 {
     switch (S) {
-        case 1: return raster_fn<1>(sub);
-        case 2: return raster_fn<2>(sub);
-        case 3: return raster_fn<3>(sub);
-        case 4: return raster_fn<4>(sub);
-        case 5: return raster_fn<5>(sub);
-        case 6: return raster_fn<6>(sub);
-        case 7: return raster_fn<7>(sub);
-        case 8: return raster_fn<8>(sub);
-        case 9: return raster_fn<9>(sub);
-        case 10: return raster_fn<10>(sub);
-        case 11: return raster_fn<11>(sub);
-        case 12: return raster_fn<12>(sub);
-        case 13: return raster_fn<13>(sub);
-        case 14: return raster_fn<14>(sub);
-        case 15: return raster_fn<15>(sub);
+        case 1: return kf_raster<1, SUB>;
+        case 2: return kf_raster<2, SUB>;
+        case 3: return kf_raster<3, SUB>;
+        case 4: return kf_raster<4, SUB>;
+        case 5: return kf_raster<5, SUB>;
+        case 6: return kf_raster<6, SUB>;
+        case 7: return kf_raster<7, SUB>;
+        case 8: return kf_raster<8, SUB>;
+        case 9: return kf_raster<9, SUB>;
+        case 10: return kf_raster<10, SUB>;
+        case 11: return kf_raster<11, SUB>;
+        case 12: return kf_raster<12, SUB>;
+        case 13: return kf_raster<13, SUB>;
+        case 14: return kf_raster<14, SUB>;
+        case 15: return kf_raster<15, SUB>;
     }
     return nullptr;
 }
 
+#ifdef DPX_RASTER_SUB_TU
+RasterFn pick_raster_sub(int S) { return pick_raster_of<true>(S); }
+#else
+RasterFn pick_raster_sub(int S);  // in the DPX_RASTER_SUB_TU build of this file
+
+RasterFn pick_raster(int S, bool sub) { return sub ? pick_raster_sub(S) : pick_raster_of<false>(S); }
+#endif
+
 }  // namespace dpx
+
+#ifndef DPX_RASTER_SUB_TU
 
 // ---------------------------------------------------------------------------
 // context
@@ -1042,3 +1062,5 @@ extern "C" int dpx_batch_timing_read(dpx_context* ctx, dpx_batch_times* out)
     ctx->acc = dpx_batch_times{};
     return DPX_OK;
 }
+
+#endif  // DPX_RASTER_SUB_TU

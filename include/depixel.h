@@ -217,8 +217,13 @@ int dpx_context_reserve(dpx_context* ctx, int max_items, int64_t max_item_pixels
  * context must be serialised by the caller (same stream or explicit sync). */
 int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_out, const dpx_item* items,
                   int n_items, void* stream);
-/* Optional resolved graphs of the last batch (device pointer + per-item
- * offsets), for parity checks. */
+/* Whether the batches that follow also write the resolved graph of every
+ * item (1 byte per pixel, the reference's graph_h): off by default, since
+ * the images are the product and the graph writes cost about 3.5% of the
+ * raster kernel's time. */
+int dpx_context_keep_graph(dpx_context* ctx, int enable);
+/* The resolved graphs of the last batch (device pointer + per-item offsets);
+ * DPX_EINVAL unless that batch ran with dpx_context_keep_graph on. */
 int dpx_batch_graph(dpx_context* ctx, const uint8_t** d_graph, const uint64_t** offsets_host);
 
 /* Per-kernel timing of the batched path: HIP events recorded on the batch's

@@ -472,8 +472,8 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                          (((B >> 1) & 0x01010101u) << 7);  // down-right block: "\"
         uint32_t res = s_aux32[(r + 1) * RWW + w] & ~clear;
         s_g32[r * RWW + w] = res;
-        // the resolved graph of the tile's own cells
-        if (y >= y0 && y < y0 + RTY && w >= 2 && w <= 9 && y < it.H && x < it.W) {
+        // the resolved graph of the tile's own cells, when the caller keeps it
+        if (gfin && y >= y0 && y < y0 + RTY && w >= 2 && w <= 9 && y < it.H && x < it.W) {
             uint8_t* dst = gfin + it.g_off + (uint32_t)(y * it.W + x);
             if (it.graph_w4) {
                 *reinterpret_cast<uint32_t*>(dst) = res;
@@ -736,6 +736,8 @@ struct dpx_context {
     bool upload_pending = false;
     std::vector<uint8_t> last_desc;
     std::vector<uint64_t> g_offsets;
+    bool keep_graph = false;  // write the resolved graphs (dpx_context_keep_graph)
+    bool last_kept = false;   // the last batch wrote them
     std::mutex mu;
     // kernel timing: 3 events per recorded batch (start, after graph, end)
     bool timing = false;
@@ -899,9 +901,18 @@ extern "C" int dpx_context_reserve(dpx_context* ctx, int max_items, int64_t max_
     return DPX_OK;
 }
 
+extern "C" int dpx_context_keep_graph(dpx_context* ctx, int enable)
+{
+    DPX_CHECK(ctx, DPX_EINVAL, "null context");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->keep_graph = enable != 0;
+    return DPX_OK;
+}
+
 extern "C" int dpx_batch_graph(dpx_context* ctx, const uint8_t** d_graph, const uint64_t** offsets_host)
 {
     DPX_CHECK(ctx && d_graph && offsets_host, DPX_EINVAL, "bad arguments");
+    DPX_CHECK(ctx->last_kept, DPX_EINVAL, "the last batch wrote no graphs (see dpx_context_keep_graph)");
     *d_graph = ctx->d_gfin;
     *offsets_host = ctx->g_offsets.data();
     return DPX_OK;
@@ -1020,16 +1031,18 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
     hipLaunchKernelGGL(kf_graph, ggrid, dim3(256), 0, stream, d_items, d_gmap, g3d, d_in, ctx->d_gaux);
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[1], stream));
     int gk = 0;
+    uint8_t* gfin = ctx->keep_graph ? ctx->d_gfin : nullptr;
     for (auto& kv : groups) {
         Group& grp = kv.second;
         RasterFn fn = pick_raster(grp.scale, grp.sub);
         DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", grp.scale);
         const DevItem* d_gitems = reinterpret_cast<const DevItem*>(ctx->d_desc + goffs[gk].first);
         const uint2* d_rmap = reinterpret_cast<const uint2*>(ctx->d_desc + goffs[gk].second);
-        hipLaunchKernelGGL(fn, grp.grid, dim3(256), 0, stream, d_gitems, d_rmap, grp.grid3d, d_in, ctx->d_gaux,
-                           ctx->d_gfin, ctx->d_hull, ctx->d_mask[grp.scale], ctx->d_info[grp.scale], d_out);
+        hipLaunchKernelGGL(fn, grp.grid, dim3(256), 0, stream, d_gitems, d_rmap, grp.grid3d, d_in, ctx->d_gaux, gfin,
+                           ctx->d_hull, ctx->d_mask[grp.scale], ctx->d_info[grp.scale], d_out);
         gk++;
     }
+    ctx->last_kept = ctx->keep_graph;
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[2], stream));
     DPX_HIP(hipGetLastError());
     return DPX_OK;

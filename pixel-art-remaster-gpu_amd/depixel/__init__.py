@@ -97,6 +97,7 @@ _SIGS = {
     "dpx_context_destroy": (None, [c_void_p]),
     "dpx_batch_run": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(Item), c_int, c_void_p]),
     "dpx_batch_graph": (c_int, [c_void_p, POINTER(c_void_p), POINTER(POINTER(c_uint64))]),
+    "dpx_context_keep_graph": (c_int, [c_void_p, c_int]),
     "dpx_batch_timing": (c_int, [c_void_p, c_int]),
     "dpx_batch_timing_read": (c_int, [c_void_p, POINTER(BatchTimes)]),
     "dpx_server_create": (c_int, [c_int, c_uint64, c_int, POINTER(c_void_p)]),
@@ -246,6 +247,10 @@ class Context:
 
     def timing(self, enable: bool):
         check(lib.dpx_batch_timing(self.h, int(enable)))
+
+    def keep_graph(self, enable: bool):
+        """Whether the following batches also write the resolved graphs (off by default)."""
+        check(lib.dpx_context_keep_graph(self.h, int(enable)))
 
     def timing_read(self) -> BatchTimes:
         t = BatchTimes()

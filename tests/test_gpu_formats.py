@@ -62,6 +62,7 @@ def run_items(torch, D, blobs_items):
     d_in = torch.from_numpy(np.concatenate(chunks)).to("cuda:0")
     d_out = torch.zeros(out_off, dtype=torch.uint8, device="cuda:0")
     ctx = D.Context(0)
+    ctx.keep_graph(True)
     ctx.run(d_in, d_out, items)
     torch.cuda.synchronize()
     out = d_out.cpu().numpy()

@@ -177,6 +177,7 @@ def run_batch(torch, D, frames, items_spec):
     d_in = torch.from_numpy(np.concatenate(blobs)).to("cuda:0")
     d_out = torch.zeros(out_off, dtype=torch.uint8, device="cuda:0")
     ctx = D.Context(0)
+    ctx.keep_graph(True)
     ctx.run(d_in, d_out, items)
     torch.cuda.synchronize()
     out = d_out.cpu().numpy()
@@ -236,6 +237,30 @@ def test_batch_result_independent_of_batch_mix(gpu, D):
     spec[2] = (4, False, 0)
     mixed, _ = run_batch(torch, D, frames, spec)
     assert_equal(mixed[2], alone[0], "frame alone vs in a batch")
+
+
+def test_graph_output_is_opt_in(gpu, D):
+    """Without dpx_context_keep_graph the batch writes the same images and
+    no graphs, and dpx_batch_graph says so."""
+    torch = gpu
+    W, H, S = 96, 64, 4
+    img = D.synth_frame(0, 91, W, H, 3)
+    d_in = torch.from_numpy(img.reshape(-1)).to("cuda:0")
+    outs = []
+    for keep in (False, True):
+        d_out = torch.zeros(H * S * W * S * 4, dtype=torch.uint8, device="cuda:0")
+        ctx = D.Context(0)
+        ctx.keep_graph(keep)
+        ctx.run(d_in, d_out, [D.Item(0, 0, W, H, img.shape[1], 3, S, 0, 0, W * S * 4)])
+        torch.cuda.synchronize()
+        outs.append(d_out.cpu().numpy())
+        if keep:
+            ctx.graph(1)
+        else:
+            with pytest.raises(D.DepixelError):
+                ctx.graph(1)
+        ctx.close()
+    assert_equal(outs[0], outs[1], "images with and without the graph output")
 
 
 def gpu_border_chains(torch, D, g, max_nodes=None, max_chains=None):

@@ -358,7 +358,8 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     constexpr int PX_WORDS = 2048, PX_CELLS = PX_WORDS / (S * S);
     static_assert(PX_CELLS >= 1, "at least one buffered cell");
     __shared__ uint32_t s_px[PX_WORDS];
-    __shared__ uint16_t s_idx[RTX * RTY];  // list index of a listed own cell
+    __shared__ uint16_t s_idx[RTX * RTY];    // list index of a listed own cell
+    __shared__ uint8_t s_simple[RTX * RTY];  // own cell shows its own colour only
     const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
 
     const int t = threadIdx.x;
@@ -594,6 +595,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         s_list[i] = (uint16_t)t;
         s_idx[t] = (uint16_t)i;
     }
+    s_simple[t] = simple ? 1 : 0;
     __syncthreads();
 
     // One unit = one output row (S pixels) of a listed cell, resolved
@@ -657,6 +659,56 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             resolve_unit(u, rows);
         }
         __syncthreads();
+    }
+
+    // At S = 1, 2 and 4 one output row of a cell is one store and the lanes
+    // of a wave store adjacent cells, so each thread stores its own cell.  At
+    // the other scales that would make every store instruction strided (S * 4
+    // bytes between lanes, each line written in pieces by several
+    // instructions), so the tile's output is stored in chunks of CW pixels
+    // inside one cell, chunk c by thread c % 256: adjacent lanes store
+    // adjacent chunks of a row.  A tile whose searched cells did not all fit
+    // the pixel buffer keeps the per-cell stores.
+    constexpr int CW = (S % 4 == 0) ? 4 : ((S % 2 == 0) ? 2 : 1);
+    bool chunked = false;
+    if constexpr (S != 1 && S != 2 && S != 4) {
+        const uintptr_t base = reinterpret_cast<uintptr_t>(out + it.out_off);
+        chunked = s_nlist <= PX_CELLS && ((base | (uintptr_t)it.out_stride) & (4 * CW - 1)) == 0;
+    }
+    if (chunked) {
+        constexpr int CPC = S / CW;          // chunks per cell row
+        constexpr int CPR = RTX * CPC;       // chunks per output row of the tile
+        constexpr int NCH = RTY * S * CPR;   // chunks of the tile
+        for (int c = t; c < NCH; c += 256) {
+            const int ry = c / CPR, rc = c % CPR;
+            const int lx = rc / CPC, kx = (rc % CPC) * CW;
+            const int ly = ry / S, ky = ry % S;
+            if (x0 + lx >= it.W || y0 + ly >= it.H) continue;
+            const int own = ly * RTX + lx;
+            uint32_t col[CW];
+            if (s_simple[own]) {
+                const uint32_t v = s_col[(ly + 1) * RCX + lx + 1];
+#pragma unroll
+                for (int q = 0; q < CW; q++) col[q] = v;
+            } else {
+                const uint32_t* p = s_px + (s_idx[own] * S + ky) * S + kx;
+#pragma unroll
+                for (int q = 0; q < CW; q++) col[q] = p[q];
+            }
+            const int Y = (y0 + ly) * S + ky;
+            const int row = out_topdown(it.flags) ? (OH - 1 - Y) : Y;
+            uint8_t* dst = out + it.out_off + (size_t)row * it.out_stride + ((size_t)(x0 + lx) * S + kx) * 4;
+            if constexpr (CW == 4) {
+                u32x4 v = { col[0], col[1], col[2], col[3] };
+                __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(dst));
+            } else if constexpr (CW == 2) {
+                u32x2 v = { col[0], col[1] };
+                __builtin_nontemporal_store(v, reinterpret_cast<u32x2*>(dst));
+            } else {
+                __builtin_nontemporal_store(col[0], reinterpret_cast<uint32_t*>(dst));
+            }
+        }
+        return;
     }
     if (simple) {
         uint32_t col[S];

@@ -773,7 +773,29 @@ RasterFn pick_raster(int S, bool sub) { return sub ? pick_raster_sub(S) : pick_r
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
+// The launch plan of the last batch.  A batch whose items and output
+// alignment equal the last one's reuses it (no validation, grouping, tile
+// maps or descriptor comparison): for thousands of sprites that host work
+// takes longer than the kernels.
+struct PlanGroup {
+    int scale;
+    bool sub;
+    dim3 grid;
+    int grid3d;
+    size_t off_items, off_map;  // in the descriptor buffer
+};
+struct BatchPlan {
+    bool valid = false;
+    std::vector<dpx_item> items;
+    uintptr_t out_mod = 0;  // d_out % 16
+    dim3 ggrid;
+    int g3d = 0;
+    size_t off_items = 0, off_gmap = 0;
+    std::vector<PlanGroup> groups;
+};
+
 struct dpx_context {
+    BatchPlan plan;
     int device = 0;
     int8_t* d_hull = nullptr;
     uint32_t* d_mask[dpx::kMaxScale + 1] = {};
@@ -857,6 +879,7 @@ int grow_desc(dpx_context* ctx, size_t need)
     ctx->h_pinned = nullptr;
     ctx->desc_cap = ctx->pinned_cap = 0;
     ctx->last_desc.clear();
+    ctx->plan.valid = false;  // its descriptors are gone with the buffer
     size_t cap = align_up(need * 2, 1 << 16);
     DPX_HIP(hipMalloc(&ctx->d_desc, cap));
     DPX_HIP(hipHostMalloc(&ctx->h_pinned, cap, hipHostMallocDefault));
@@ -970,15 +993,14 @@ extern "C" int dpx_batch_graph(dpx_context* ctx, const uint8_t** d_graph, const 
     return DPX_OK;
 }
 
-extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_out, const dpx_item* items,
-                             int n_items, void* stream_)
+// Validates the items and builds the launch plan: device items, graph tiles,
+// raster groups by (scale, subdivide) and the descriptor blob, uploaded on
+// `stream` when it differs from the one already on the device.
+static int make_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* items, int n_items, hipStream_t stream)
 {
-    DPX_CHECK(ctx && d_in && d_out && items && n_items > 0, DPX_EINVAL, "bad arguments");
-    std::lock_guard<std::mutex> lock(ctx->mu);
-    hipStream_t stream = as_stream(stream_);
-    DPX_HIP(hipSetDevice(ctx->device));
-
-    // plan: device items, graph tiles, raster groups by (scale, subdivide)
+    using namespace dpx;
+    BatchPlan& plan = ctx->plan;
+    plan.valid = false;
     std::vector<DevItem> dev(n_items);
     ctx->g_offsets.assign(n_items, 0);
     std::map<int, Group> groups;
@@ -1066,8 +1088,37 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         ctx->upload_pending = true;
         ctx->last_desc.swap(blob);
     }
-    const DevItem* d_items = reinterpret_cast<const DevItem*>(ctx->d_desc + off_items);
-    const uint2* d_gmap = reinterpret_cast<const uint2*>(ctx->d_desc + off_gmap);
+    plan.ggrid = ggrid;
+    plan.g3d = g3d;
+    plan.off_items = off_items;
+    plan.off_gmap = off_gmap;
+    plan.groups.clear();
+    size_t gi = 0;
+    for (auto& kv : groups) {
+        const Group& grp = kv.second;
+        plan.groups.push_back(PlanGroup{ grp.scale, grp.sub, grp.grid, grp.grid3d, goffs[gi].first, goffs[gi].second });
+        gi++;
+    }
+    plan.items.assign(items, items + n_items);
+    plan.out_mod = reinterpret_cast<uintptr_t>(d_out) % 16;
+    plan.valid = true;
+    return DPX_OK;
+}
+
+extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_out, const dpx_item* items,
+                             int n_items, void* stream_)
+{
+    DPX_CHECK(ctx && d_in && d_out && items && n_items > 0, DPX_EINVAL, "bad arguments");
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    hipStream_t stream = as_stream(stream_);
+    DPX_HIP(hipSetDevice(ctx->device));
+    const BatchPlan& plan = ctx->plan;
+    if (!(plan.valid && plan.items.size() == (size_t)n_items &&
+          plan.out_mod == reinterpret_cast<uintptr_t>(d_out) % 16 &&
+          memcmp(plan.items.data(), items, (size_t)n_items * sizeof(dpx_item)) == 0))
+        if (int rc = make_plan(ctx, d_out, items, n_items, stream)) return rc;
+    const DevItem* d_items = reinterpret_cast<const DevItem*>(ctx->d_desc + plan.off_items);
+    const uint2* d_gmap = reinterpret_cast<const uint2*>(ctx->d_desc + plan.off_gmap);
 
     hipEvent_t ev[3] = {};
     if (ctx->timing) {
@@ -1080,19 +1131,16 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         ctx->ev_used += 3;
         DPX_HIP(hipEventRecord(ev[0], stream));
     }
-    hipLaunchKernelGGL(kf_graph, ggrid, dim3(256), 0, stream, d_items, d_gmap, g3d, d_in, ctx->d_gaux);
+    hipLaunchKernelGGL(kf_graph, plan.ggrid, dim3(256), 0, stream, d_items, d_gmap, plan.g3d, d_in, ctx->d_gaux);
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[1], stream));
-    int gk = 0;
     uint8_t* gfin = ctx->keep_graph ? ctx->d_gfin : nullptr;
-    for (auto& kv : groups) {
-        Group& grp = kv.second;
-        RasterFn fn = pick_raster(grp.scale, grp.sub);
-        DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", grp.scale);
-        const DevItem* d_gitems = reinterpret_cast<const DevItem*>(ctx->d_desc + goffs[gk].first);
-        const uint2* d_rmap = reinterpret_cast<const uint2*>(ctx->d_desc + goffs[gk].second);
-        hipLaunchKernelGGL(fn, grp.grid, dim3(256), 0, stream, d_gitems, d_rmap, grp.grid3d, d_in, ctx->d_gaux, gfin,
-                           ctx->d_hull, ctx->d_mask[grp.scale], ctx->d_info[grp.scale], d_out);
-        gk++;
+    for (const PlanGroup& pg : plan.groups) {
+        RasterFn fn = pick_raster(pg.scale, pg.sub);
+        DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", pg.scale);
+        const DevItem* d_gitems = reinterpret_cast<const DevItem*>(ctx->d_desc + pg.off_items);
+        const uint2* d_rmap = reinterpret_cast<const uint2*>(ctx->d_desc + pg.off_map);
+        hipLaunchKernelGGL(fn, pg.grid, dim3(256), 0, stream, d_gitems, d_rmap, pg.grid3d, d_in, ctx->d_gaux, gfin,
+                           ctx->d_hull, ctx->d_mask[pg.scale], ctx->d_info[pg.scale], d_out);
     }
     ctx->last_kept = ctx->keep_graph;
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[2], stream));

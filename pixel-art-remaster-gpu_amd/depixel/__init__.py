@@ -98,6 +98,7 @@ _SIGS = {
     "dpx_batch_run": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(Item), c_int, c_void_p]),
     "dpx_batch_graph": (c_int, [c_void_p, POINTER(c_void_p), POINTER(POINTER(c_uint64))]),
     "dpx_context_keep_graph": (c_int, [c_void_p, c_int]),
+    "dpx_context_reserve": (c_int, [c_void_p, c_int, ctypes.c_int64, c_int]),
     "dpx_batch_timing": (c_int, [c_void_p, c_int]),
     "dpx_batch_timing_read": (c_int, [c_void_p, POINTER(BatchTimes)]),
     "dpx_server_create": (c_int, [c_int, c_uint64, c_int, POINTER(c_void_p)]),
@@ -224,6 +225,11 @@ def launch_kernel(img: np.ndarray, width: int, height: int, subdivide: bool = Fa
 # --------------------------------------------------------------------------
 # batched path
 # --------------------------------------------------------------------------
+def items_array(items: list[Item]):
+    """The C array of dpx_item that dpx_batch_run takes."""
+    return (Item * len(items))(*items)
+
+
 class Context:
     def __init__(self, device: int = 0):
         h = c_void_p()
@@ -241,9 +247,10 @@ class Context:
         except Exception:
             pass
 
-    def run(self, d_in, d_out, items: list[Item], stream=None):
-        arr = (Item * len(items))(*items)
-        check(lib.dpx_batch_run(self.h, _ptr(d_in), _ptr(d_out), arr, len(items), _stream(stream)))
+    def run(self, d_in, d_out, items, stream=None):
+        """items: a list of Item, or an `items_array(...)` built once for repeated batches."""
+        arr = items if isinstance(items, ctypes.Array) else items_array(items)
+        check(lib.dpx_batch_run(self.h, _ptr(d_in), _ptr(d_out), arr, len(arr), _stream(stream)))
 
     def timing(self, enable: bool):
         check(lib.dpx_batch_timing(self.h, int(enable)))

@@ -263,6 +263,38 @@ def test_graph_output_is_opt_in(gpu, D):
     assert_equal(outs[0], outs[1], "images with and without the graph output")
 
 
+def test_repeated_and_changed_batches_on_one_context(gpu, D, O):
+    """A context reuses the last batch's plan only for identical items: a
+    repeat, an item pointed at another frame, another scale, and a reserve
+    that replaces the descriptor buffer in between all give the oracle's
+    output."""
+    torch = gpu
+    W, H = 64, 40
+    frames = [D.synth_frame(0, 120 + k, W, H, 3) for k in range(3)]
+    fb = frames[0].size
+    d_in = torch.from_numpy(np.concatenate([f.reshape(-1) for f in frames])).to("cuda:0")
+    ctx = D.Context(0)
+
+    def run(items, S):
+        d_out = torch.zeros(len(items) * H * S * W * S * 4, dtype=torch.uint8, device="cuda:0")
+        ctx.run(d_in, d_out, items)
+        torch.cuda.synchronize()
+        return d_out.cpu().numpy().reshape(len(items), H * S, W * S, 4)
+
+    def items_for(order, S):
+        return [D.Item(k * fb, i * H * S * W * S * 4, W, H, W * 3, 3, S, 0, 0, W * S * 4) for i, k in enumerate(order)]
+
+    refs = {(k, S): O.depixelize(frames[k], W, H, scale=S)["rgba"] for k in range(3) for S in (3, 4)}
+    for order, S, reserve in (((0, 1), 4, False), ((0, 1), 4, False), ((2, 1), 4, False), ((2, 1), 3, False),
+                              ((2, 1), 3, True), ((0, 2), 4, False)):
+        if reserve:
+            D.check(D.lib.dpx_context_reserve(ctx.h, 4096, W * H, 3))
+        got = run(items_for(order, S), S)
+        for i, k in enumerate(order):
+            assert_equal(got[i], refs[(k, S)], f"order {order} scale {S} item {i}")
+    ctx.close()
+
+
 def gpu_border_chains(torch, D, g, max_nodes=None, max_chains=None):
     H, W = g.shape
     dev = "cuda:0"

@@ -203,21 +203,22 @@ __device__ inline TileAt locate_tile(const uint2* __restrict__ tile_map, int gri
 // ---------------------------------------------------------------------------
 // Three passes over LDS:
 //   1. lane words (pixel class + YUV fields, see lane_ref) of the tile and a
-//      halo: x0-1 .. x0+64, y0-2 .. y0+33;
-//   2. per position of x0-1 .. x0+63, y0-1 .. y0+32, the four edges it owns
+//      halo: x0-1 .. x0+65, y0-2 .. y0+33;
+//   2. per position of x0-1 .. x0+64, y0-1 .. y0+32, the four edges it owns
 //      (up, up-right, right, down-right: bits 1, 2, 4, 7; similarity is
-//      symmetric, so each edge is tested once) and in bit 0 whether the 2x2
-//      block whose lower-left pixel it is has all four orthogonal edges
-//      ("full"), which is crossCheck_4's condition for removing both
-//      diagonals;
+//      symmetric, so each edge is tested once);
 //   3. per four pixels of a row at once (one byte each in a 32-bit word):
 //      the other four edges from the left, down, down-left and up-left
-//      neighbours' bytes and the trivial removal from the full flags of the
-//      four blocks around the pixel.
+//      neighbours' bytes, and the trivial removal: a 2x2 block is "full"
+//      (all four orthogonal edges, crossCheck_4's condition for removing
+//      both diagonals) iff its lower-left position has its up and right
+//      edges, the position above it its right edge and the position right
+//      of it its up edge; the flags of the four blocks around a pixel come
+//      from those bits of the neighbouring bytes.
 constexpr int GTX = 64, GTY = 32;
-constexpr int GLX = GTX + 2, GLY = GTY + 4;  // lane words
-constexpr int GHX = GTX + 1, GHY = GTY + 2;  // positions of pass 2
-constexpr int GHP = GTX + 4;                 // byte pitch of pass 2's rows: column c is x = x0 - 4 + c
+constexpr int GLX = GTX + 3, GLY = GTY + 4;  // lane words
+constexpr int GHX = GTX + 2, GHY = GTY + 2;  // positions of pass 2
+constexpr int GHP = GTX + 8;                 // byte pitch of pass 2's rows: column c is x = x0 - 4 + c
 
 __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
                                                 int grid3d, const uint8_t* __restrict__ in,
@@ -269,10 +270,9 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         int cx = k % GHX, cy = k / GHX;
         int c = (cy + 1) * GLX + cx;  // lane word of position (x0-1+cx, y0-1+cy)
         uint32_t up = s_lane[c + GLX], ur = s_lane[c + GLX + 1], rt = s_lane[c + 1], dr = s_lane[c - GLX + 1];
-        LaneRef a = lane_ref(s_lane[c]), b = lane_ref(ur);
-        uint32_t u = lane_similar(a, up), r = lane_similar(a, rt);
-        uint32_t h = u << 1 | lane_similar(a, ur) << 2 | r << 4 | lane_similar(a, dr) << 7;
-        h |= u & r & lane_similar(b, up) & lane_similar(b, rt);
+        LaneRef a = lane_ref(s_lane[c]);
+        uint32_t h = lane_similar(a, up) << 1 | lane_similar(a, ur) << 2 | lane_similar(a, rt) << 4 |
+                     lane_similar(a, dr) << 7;
         s_h[cy * GHP + cx + 3] = (uint8_t)h;
     }
     __syncthreads();
@@ -290,8 +290,19 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         uint32_t ul = __builtin_amdgcn_alignbyte(up[1], up[0], 3);
         uint32_t edges = (own & 0x96969696u) | ((left & 0x10101010u) >> 1) | ((down & 0x02020202u) << 5) |
                          ((dl & 0x04040404u) << 3) | ((ul & 0x80808080u) >> 7);
-        uint32_t clear = ((own & 0x01010101u) << 2) | (left & 0x01010101u) | ((dl & 0x01010101u) << 5) |
-                         ((down & 0x01010101u) << 7);
+        // full flags (bit 0 of each byte) of the blocks whose lower-left
+        // positions are the bytes of h: its up (bit 1) and right (bit 4)
+        // edges, the right edge of the byte above (in hu) and the up edge of
+        // the byte to the right (in hr, h shifted by one position)
+        auto full = [](uint32_t h, uint32_t hr, uint32_t hu) {
+            return (h >> 1) & (h >> 4) & (hu >> 4) & (hr >> 1) & 0x01010101u;
+        };
+        const uint32_t fm1 = full(md[1], __builtin_amdgcn_alignbyte(md[2], md[1], 1), up[1]);  // x .. x+3, row y
+        const uint32_t fm0 = full(md[0], __builtin_amdgcn_alignbyte(md[1], md[0], 1), up[0]);  // byte 3: x-1
+        const uint32_t fd1 = full(dn[1], __builtin_amdgcn_alignbyte(dn[2], dn[1], 1), md[1]);  // row y-1
+        const uint32_t fd0 = full(dn[0], __builtin_amdgcn_alignbyte(dn[1], dn[0], 1), md[0]);
+        uint32_t clear = (fm1 << 2) | __builtin_amdgcn_alignbyte(fm1, fm0, 3) |
+                         (__builtin_amdgcn_alignbyte(fd1, fd0, 3) << 5) | (fd1 << 7);
         uint32_t v = edges & ~clear;
         uint8_t* dst = g + (uint32_t)(y * it.W + x);
         if (it.graph_w4) {

@@ -678,13 +678,16 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     // bytes between lanes, each line written in pieces by several
     // instructions), so the tile's output is stored in chunks of CW pixels
     // inside one cell, chunk c by thread c % 256: adjacent lanes store
-    // adjacent chunks of a row.  A tile whose searched cells did not all fit
-    // the pixel buffer keeps the per-cell stores.
+    // adjacent chunks of a row.  Searched cells beyond the pixel buffer were
+    // stored by the search itself; their chunks are skipped.  Measured: with
+    // chunks of fewer than 4 pixels the chunk loop's own work outweighs the
+    // coalescing once cells overflow (dense 5x frames, 6x sprites), so such
+    // tiles keep the per-cell stores; at 8x overflowing tiles gain 15%.
     constexpr int CW = (S % 4 == 0) ? 4 : ((S % 2 == 0) ? 2 : 1);
     bool chunked = false;
     if constexpr (S != 1 && S != 2 && S != 4) {
         const uintptr_t base = reinterpret_cast<uintptr_t>(out + it.out_off);
-        chunked = s_nlist <= PX_CELLS && ((base | (uintptr_t)it.out_stride) & (4 * CW - 1)) == 0;
+        chunked = ((base | (uintptr_t)it.out_stride) & (4 * CW - 1)) == 0 && (CW == 4 || s_nlist <= PX_CELLS);
     }
     if (chunked) {
         constexpr int CPC = S / CW;          // chunks per cell row
@@ -702,7 +705,9 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
 #pragma unroll
                 for (int q = 0; q < CW; q++) col[q] = v;
             } else {
-                const uint32_t* p = s_px + (s_idx[own] * S + ky) * S + kx;
+                const int i = s_idx[own];
+                if (i >= PX_CELLS) continue;  // stored by the search
+                const uint32_t* p = s_px + (i * S + ky) * S + kx;
 #pragma unroll
                 for (int q = 0; q < CW; q++) col[q] = p[q];
             }

@@ -263,6 +263,44 @@ def test_graph_output_is_opt_in(gpu, D):
     assert_equal(outs[0], outs[1], "images with and without the graph output")
 
 
+def test_unaligned_and_padded_output_layouts(gpu, D, O):
+    """Output offsets that are 4 mod 16 and padded output rows take the
+    narrower store paths (per-cell rows, 8- or 4-byte stores); the pixels
+    and the padding around them must come out exactly."""
+    torch = gpu
+    cases = [(0, 131, 40, 24, 3, 0), (4, 132, 37, 21, 6, 2), (2, 133, 33, 17, 8, 8), (4, 134, 30, 18, 12, 0),
+             (1, 135, 24, 24, 6, 4 | 8)]
+    frames, items, refs = [], [], []
+    in_off, out_off = 0, 4  # every output starts 4 bytes past a 16-byte boundary
+    for kind, seed, W, H, S, flags in cases:
+        ch = 4 if kind == 1 else 3
+        img = D.synth_frame(kind, seed, W, H, ch)
+        out_stride = W * S * 4 + 20  # padded rows
+        items.append(D.Item(in_off, out_off, W, H, img.shape[1], ch, S, 0, flags, out_stride))
+        frames.append(img.reshape(-1))
+        in_off += img.size
+        out_off += out_stride * H * S + 16
+        refs.append(O.depixelize(img, W, H, scale=S, channels=ch, flags=flags & ~8)["rgba"])
+    d_in = torch.from_numpy(np.concatenate(frames)).to("cuda:0")
+    sentinel = 0xA5
+    d_out = torch.full((out_off,), sentinel, dtype=torch.uint8, device="cuda:0")
+    ctx = D.Context(0)
+    ctx.run(d_in, d_out, items)
+    torch.cuda.synchronize()
+    out = d_out.cpu().numpy()
+    ctx.close()
+    covered = np.zeros(out_off, dtype=bool)
+    for it, ref, c in zip(items, refs, cases):
+        W, H, S = it.width, it.height, it.scale
+        rows = out[it.out_offset:it.out_offset + it.out_stride * H * S].reshape(H * S, it.out_stride)
+        got = rows[:, :W * S * 4].reshape(H * S, W * S, 4)
+        want = ref[::-1] if c[5] & 8 else ref
+        assert_equal(got, want, f"case {c}")
+        for y in range(H * S):
+            covered[it.out_offset + y * it.out_stride:it.out_offset + y * it.out_stride + W * S * 4] = True
+    assert (out[~covered] == sentinel).all(), "bytes outside the images were written"
+
+
 def test_repeated_and_changed_batches_on_one_context(gpu, D, O):
     """A context reuses the last batch's plan only for identical items: a
     repeat, an item pointed at another frame, another scale, and a reserve

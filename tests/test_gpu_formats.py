@@ -151,6 +151,15 @@ def test_indexed_server_stream_uses_the_palette_of_each_submit(gpu, D, O):
     ref_b = O.depixelize(exp_b, W, H, scale=s, channels=4)["rgba"]
     for k, o in enumerate(outs):
         assert_equal(o.reshape(H * s, W * s, 4), ref_a if k < 3 else ref_b, f"frame {k}")
+    # frames and results in pinned blocks move by DMA; the palette still travels with each frame
+    pin_in = D.pinned_empty(idx.shape)
+    pin_in[:] = idx
+    pins = [D.pinned_empty((H * s, W * s * 4)) for _ in range(4)]
+    tickets = [srv.submit(sid, pin_in, p) for p in pins]
+    for t in tickets:
+        assert srv.wait(t) == 0
+    for k, p in enumerate(pins):
+        assert_equal(p.reshape(H * s, W * s, 4), ref_b, f"pinned frame {k}")
     srv.close()
 
 

@@ -100,6 +100,42 @@ __device__ inline uint32_t coverage_row(const int* x, const int* y, const uint8_
     return bits;
 }
 
+// All mask_w(S) rows of coverage_row at once, with each triangle's vertices
+// read and oriented once (the polygon arrays live in memory, not registers,
+// so coverage_row's per-row re-reads dominate).  Same bits as coverage_row.
+template <int S>
+__device__ inline void coverage_rows(const int* x, const int* y, const uint8_t* tri, int nt, uint32_t* rows)
+{
+    constexpr int kmin = mask_kmin(S), mw = mask_w(S);
+#pragma unroll
+    for (int r = 0; r < mw; r++) rows[r] = 0;
+    for (int t = 0; t < nt; t++) {
+        int ax = x[tri[3 * t]] * 2 * S, ay = y[tri[3 * t]] * 2 * S;
+        int bx = x[tri[3 * t + 1]] * 2 * S, by = y[tri[3 * t + 1]] * 2 * S;
+        int cx = x[tri[3 * t + 2]] * 2 * S, cy = y[tri[3 * t + 2]] * 2 * S;
+        const int o = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
+        if (o == 0) continue;  // degenerate: covers nothing (tri_covers)
+        if (o < 0) {
+            int tx = bx, ty = by;
+            bx = cx; by = cy; cx = tx; cy = ty;
+        }
+        const int ymin = min(ay, min(by, cy)), ymax = max(ay, max(by, cy));
+#pragma unroll
+        for (int r = 0; r < mw; r++) {
+            const int sy = (2 * (kmin + r) + 1) * 64;
+            if (sy < ymin || sy > ymax) continue;
+            uint32_t bits = 0;
+            for (int b = 0; b < mw; b++) {
+                const int sx = (2 * (kmin + b) + 1) * 64;
+                if (edge_in(ax, ay, bx, by, sx, sy) && edge_in(bx, by, cx, cy, sx, sy) &&
+                    edge_in(cx, cy, ax, ay, sx, sy))
+                    bits |= 1u << b;
+            }
+            rows[r] |= bits;
+        }
+    }
+}
+
 #ifndef DPX_RASTER_SUB_TU
 __global__ void kt_mask(const int8_t* __restrict__ hull_table, int S, uint32_t* __restrict__ mask)
 {
@@ -553,10 +589,9 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             int ok;
             int nt = ear_clip(ox, oy, m, tri, &ok);
             uint32_t rows[MW];
-            for (int r = 0; r < MW; r++) {
-                rows[r] = coverage_row(ox, oy, tri, nt, S, r);
-                s_mask[k * MW + r] = rows[r];
-            }
+            coverage_rows<S>(ox, oy, tri, nt, rows);
+#pragma unroll
+            for (int r = 0; r < MW; r++) s_mask[k * MW + r] = rows[r];
             s_info[k] = (uint8_t)(coverage_info(rows, S) & 0x1Fu);
         }
     }

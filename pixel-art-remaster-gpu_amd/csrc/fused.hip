@@ -284,33 +284,49 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         s_h32[t] = lw;
         __syncthreads();
     }
-    for (int k = t; k < GLX * GLY; k += 256) {
-        int x = x0 - 1 + k % GLX, y = y0 - 2 + k / GLX;
-        uint32_t lw = 0;
-        if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
-            int row = topdown ? it.H - 1 - y : y;
-            uint32_t off = __umul24(row, it.stride) + __umul24(x, it.ch);  // operands < 2^24
-            if (indexed) {
-                lw = s_h32[base[off]];
-            } else {
-                int c0 = base[off + o0], c1 = base[off + 1], c2 = base[off + o2];
-                lw = yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque;
-                if (alpha && base[off + 3] == 0) lw = kLaneTransparent;
-            }
+    // Lane word of pixel (x, y) at byte offset `off` of the item's buffer.
+    auto lane_word = [&](uint32_t off) -> uint32_t {
+        if (indexed) return s_h32[base[off]];
+        int c0 = base[off + o0], c1 = base[off + 1], c2 = base[off + o2];
+        if (alpha && base[off + 3] == 0) return kLaneTransparent;
+        return yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque;
+    };
+    // Window column c (x = x0 - 1 + c), row r (y = y0 - 2 + r).  Columns
+    // 0..63: one per thread (t % 64), every fourth row from t / 64, the byte
+    // offset stepping by four rows; columns 64..GLX-1: one word per thread.
+    {
+        const int c = t & 63, x = x0 - 1 + c;
+        const bool xin = (unsigned)x < (unsigned)it.W;
+        int y = y0 - 2 + (t >> 6);
+        // modulo 2^32 (rows outside the frame may lie outside int); used only inside the frame
+        uint32_t off = (uint32_t)(topdown ? it.H - 1 - y : y) * (uint32_t)it.stride + (uint32_t)(x * it.ch);
+        const uint32_t step = (uint32_t)((topdown ? -4 : 4) * it.stride);
+        for (int r = t >> 6; r < GLY; r += 4, y += 4, off += step) {
+            uint32_t lw = 0;
+            if (xin && (unsigned)y < (unsigned)it.H) lw = lane_word(off);
+            s_lane[r * GLX + c] = lw;
         }
-        s_lane[k] = lw;
+    }
+    if (t < (GLX - 64) * GLY) {
+        const int c = 64 + t % (GLX - 64), r = t / (GLX - 64);
+        const int x = x0 - 1 + c, y = y0 - 2 + r;
+        uint32_t lw = 0;
+        if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
+            lw = lane_word((uint32_t)((topdown ? it.H - 1 - y : y) * it.stride + x * it.ch));
+        s_lane[r * GLX + c] = lw;
     }
     __syncthreads();
-    // pass 2
-    for (int k = t; k < GHX * GHY; k += 256) {
-        int cx = k % GHX, cy = k / GHX;
-        int c = (cy + 1) * GLX + cx;  // lane word of position (x0-1+cx, y0-1+cy)
+    // pass 2: position (x0 - 1 + cx, y0 - 1 + cy), same column split
+    auto edges = [&](int cx, int cy) {
+        const int c = (cy + 1) * GLX + cx;  // its lane word
         uint32_t up = s_lane[c + GLX], ur = s_lane[c + GLX + 1], rt = s_lane[c + 1], dr = s_lane[c - GLX + 1];
         LaneRef a = lane_ref(s_lane[c]);
         uint32_t h = lane_similar(a, up) << 1 | lane_similar(a, ur) << 2 | lane_similar(a, rt) << 4 |
                      lane_similar(a, dr) << 7;
         s_h[cy * GHP + cx + 3] = (uint8_t)h;
-    }
+    };
+    for (int cy = t >> 6; cy < GHY; cy += 4) edges(t & 63, cy);
+    if (t < (GHX - 64) * GHY) edges(64 + t % (GHX - 64), t / (GHX - 64));
     __syncthreads();
     // pass 3
     uint8_t* g = gaux + it.g_off;

@@ -100,6 +100,60 @@ __device__ inline uint32_t coverage_row(const int* x, const int* y, const uint8_
     return bits;
 }
 
+// ear_clip (geom.hpp) on a compacted copy of the polygon: each vertex packed
+// into one word (x in the low, y in the high 16 bits; 1/64 units fit) next
+// to its original index, removed vertices shifted out.  A vertex scan reads
+// one word per vertex instead of an index and two coordinates: the arrays
+// live in scratch memory here.  Same triangles, same failure rule.
+__device__ inline int ear_clip_compact(const int* x, const int* y, int n, uint8_t* tri)
+{
+    if (n < 3 || n > kMaxPoly) return 0;
+    uint32_t P[kMaxPoly];
+    uint8_t I[kMaxPoly];
+    int a2 = 0;
+    for (int p = n - 1, q = 0; q < n; p = q++) a2 += x[p] * y[q] - x[q] * y[p];
+    for (int v = 0; v < n; v++) {
+        const int k = a2 > 0 ? v : n - 1 - v;
+        P[v] = ((uint32_t)x[k] & 0xFFFFu) | ((uint32_t)y[k] << 16);
+        I[v] = (uint8_t)k;
+    }
+    auto px = [](uint32_t p) { return (int)(int16_t)(p & 0xFFFFu); };
+    auto py = [](uint32_t p) { return (int)p >> 16; };
+    int nv = n, ntri = 0, count = 2 * nv;
+    for (int v = nv - 1; nv > 2;) {
+        if (0 >= (count--)) return ntri;
+        int u = v;
+        if (nv <= u) u = 0;
+        v = u + 1;
+        if (nv <= v) v = 0;
+        int w = v + 1;
+        if (nv <= w) w = 0;
+        const uint32_t A = P[u], B = P[v], Cc = P[w];
+        const int ax = px(A), ay = py(A), bx = px(B), by = py(B), cx = px(Cc), cy = py(Cc);
+        bool ear = ((bx - ax) * (cy - ay) - (by - ay) * (cx - ax)) > 0;
+        if (ear) {
+            for (int p = 0; p < nv; p++) {
+                if (p == u || p == v || p == w) continue;
+                const uint32_t Q = P[p];
+                if (inside_tri(ax, ay, bx, by, cx, cy, px(Q), py(Q))) { ear = false; break; }
+            }
+        }
+        if (ear) {
+            tri[3 * ntri + 0] = I[u];
+            tri[3 * ntri + 1] = I[v];
+            tri[3 * ntri + 2] = I[w];
+            ntri++;
+            for (int s = v, t = v + 1; t < nv; s++, t++) {
+                P[s] = P[t];
+                I[s] = I[t];
+            }
+            nv--;
+            count = 2 * nv;
+        }
+    }
+    return ntri;
+}
+
 // All mask_w(S) rows of coverage_row at once, with each triangle's vertices
 // read and oriented once (the polygon arrays live in memory, not registers,
 // so coverage_row's per-row re-reads dominate).  Same bits as coverage_row.
@@ -602,8 +656,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             int ox[kMaxPoly], oy[kMaxPoly];
             int m = subdivide_cell(hx, hy, n, node, nbr, tj, ox, oy);
             uint8_t tri[3 * kMaxPoly];
-            int ok;
-            int nt = ear_clip(ox, oy, m, tri, &ok);
+            int nt = ear_clip_compact(ox, oy, m, tri);
             uint32_t rows[MW];
             coverage_rows<S>(ox, oy, tri, nt, rows);
 #pragma unroll

@@ -402,3 +402,46 @@ def test_large_same_size_batch_matches_oracle(gpu, D, O):
         ref = O.depixelize(img, W, H, scale=S, flags=spec[k][2])
         assert_equal(graphs[k], ref["graph"], f"item {k} graph")
         assert_equal(outs[k], ref["rgba"], f"item {k} rgba")
+
+
+def test_full_headline_batch(gpu, D, O):
+    """The measured workload at its full size: 1024 frames of 256x224 -> 4x in
+    one batch.  Sampled frames equal the oracle; every frame equals the same
+    frame run in a batch of its own (each frame's result depends on that frame
+    only); two runs of the batch are identical."""
+    torch = gpu
+    W, H, S, B = 256, 224, 4, 1024
+    inb, outb = W * H * 3, W * S * H * S * 4
+    host = np.stack([D.synth_frame(0, k, W, H, 3, W * 3) for k in range(B)])
+    d_in = torch.from_numpy(host.reshape(-1)).to("cuda:0")
+    items = D.items_array([D.Item(k * inb, k * outb, W, H, W * 3, 3, S, 0, 0, W * S * 4) for k in range(B)])
+    ctx = D.Context(0)
+    d_out = torch.empty(B * outb, dtype=torch.uint8, device="cuda:0")
+    ctx.run(d_in, d_out, items)
+    first = d_out.clone()
+    ctx.run(d_in, d_out, items)
+    torch.cuda.synchronize()
+    assert torch.equal(first, d_out), "two runs of the same batch differ"
+    out = d_out.view(B, H * S, W * S, 4)
+    for k in (0, 1, 511, 777, 1023):
+        print(f"frame {k}", flush=True)
+        assert_equal(out[k].cpu().numpy(), O.depixelize(host[k], W, H, scale=S)["rgba"], f"frame {k} vs oracle")
+    one = torch.empty(outb, dtype=torch.uint8, device="cuda:0")
+    ctx2 = D.Context(0)
+    for k in range(0, B, 97):
+        ctx2.run(d_in, one, [D.Item(k * inb, 0, W, H, W * 3, 3, S, 0, 0, W * S * 4)])
+        torch.cuda.synchronize()
+        assert torch.equal(one.view(H * S, W * S, 4), out[k]), f"frame {k} alone vs in the batch"
+    ctx.close()
+    ctx2.close()
+
+
+def test_large_frame_at_8x(gpu, D, O):
+    """One 640x480 frame of the dense-crossing kind at 8x (5120x3840 out)."""
+    torch = gpu
+    W, H, S = 640, 480, 8
+    img = D.synth_frame(4, 7, W, H, 3, W * 3)
+    outs, graphs = run_batch(torch, D, [(img, W, H, 3)], [(S, False, 0)])
+    ref = O.depixelize(img, W, H, scale=S)
+    assert_equal(graphs[0], ref["graph"], "graph")
+    assert_equal(outs[0], ref["rgba"], "rgba")

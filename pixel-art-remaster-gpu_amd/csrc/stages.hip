@@ -306,6 +306,71 @@ __device__ inline uint32_t present_sample(const ImgView& im, const dpx_point* __
     return 0xFF000000u;  // clear colour (main.cpp:260)
 }
 
+// The four antialiasing samples of window pixel (X, Y) in one scan: the
+// cells around all four, in reverse draw order, each triangle read once and
+// tested against every sample not yet covered.  A cell reaches at most a
+// quarter cell beyond its square, so the cells outside a sample's own 3x3
+// never cover it and the first cover found is the sample's last in draw
+// order, as in present_sample.
+__device__ inline void present_samples4(const ImgView& im, const dpx_point* __restrict__ diagram,
+                                        const int32_t* __restrict__ edge_count, const ViewGeom& v, int X, int Y,
+                                        const int* sox, const int* soy, uint32_t* col)
+{
+    const long long unit = 512LL * v.sc;
+    long long px[4], py[4];
+    long long i0 = 0, i1 = 0, j0 = 0, j1 = 0;
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        px[s] = (8LL * (X - v.tx) + sox[s]) * 6400;
+        py[s] = (8LL * (Y - v.ty) + soy[s]) * 6400;
+        col[s] = 0xFF000000u;  // clear colour (main.cpp:260)
+        const long long ci = floor_div(px[s], unit), cj = floor_div(py[s], unit);
+        if (s == 0 || ci < i0) i0 = ci;
+        if (s == 0 || ci > i1) i1 = ci;
+        if (s == 0 || cj < j0) j0 = cj;
+        if (s == 0 || cj > j1) j1 = cj;
+    }
+    // the window may lie far outside the image: clamp to the frame
+    j0 = j0 - 1 < 0 ? 0 : j0 - 1;
+    i0 = i0 - 1 < 0 ? 0 : i0 - 1;
+    j1 = j1 + 1 > im.H - 1 ? im.H - 1 : j1 + 1;
+    i1 = i1 + 1 > im.W - 1 ? im.W - 1 : i1 + 1;
+    const int cc = colour_channels(im);
+    uint32_t left = 0xFu;
+    for (long long j = j1; j >= j0 && left; j--)
+        for (long long i = i1; i >= i0 && left; i--) {
+            size_t k = (size_t)j * im.W + (size_t)i;
+            int nt = edge_count[k] - 2;
+            if (nt > kCellSize / 3) nt = kCellSize / 3;
+            const dpx_point* c = &diagram[k * kCellSize];
+            const float fi = (float)i, fj = (float)j;
+            long long sx[4], sy[4];
+#pragma unroll
+            for (int s = 0; s < 4; s++) { sx[s] = px[s] - i * unit; sy[s] = py[s] - j * unit; }
+            uint32_t hit = 0;
+            for (int t = 0; t < nt && (left & ~hit); t++) {
+                long long w[6];
+#pragma unroll
+                for (int q = 0; q < 3; q++) {
+                    dpx_point p = c[3 * t + q];
+                    w[2 * q] = (long long)(int)((p.x - fi) * 64.0f) * 8 * v.sc;
+                    w[2 * q + 1] = (long long)(int)((p.y - fj) * 64.0f) * 8 * v.sc;
+                }
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+                    if (((left & ~hit) >> s) & 1u)
+                        if (tri_covers(w[0], w[1], w[2], w[3], w[4], w[5], sx[s], sy[s])) hit |= 1u << s;
+            }
+            if (hit) {
+                const uint32_t rgba = cell_rgba(load_px(im, (int)i, (int)j), cc);
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+                    if ((hit >> s) & 1u) col[s] = rgba;
+                left &= ~hit;
+            }
+        }
+}
+
 __global__ void k_present(ImgView im, const dpx_point* __restrict__ diagram, const int32_t* __restrict__ edge_count,
                           ViewGeom v, uint8_t* __restrict__ out, int out_stride)
 {
@@ -318,10 +383,16 @@ __global__ void k_present(ImgView im, const dpx_point* __restrict__ diagram, con
     } else {
         // rotated 4x grid, averaged per channel with halves rounded up
         const int sox[4] = { 3, 7, 1, 5 }, soy[4] = { 1, 3, 5, 7 };
-        uint32_t sum[4] = { 0, 0, 0, 0 };
+        uint32_t sum[4] = { 0, 0, 0, 0 }, cs[4];
+        if (v.mode == DPX_VIEW_QUADS) {
+#pragma unroll
+            for (int s = 0; s < 4; s++) cs[s] = present_sample(im, diagram, edge_count, v, X, Y, sox[s], soy[s]);
+        } else {
+            present_samples4(im, diagram, edge_count, v, X, Y, sox, soy, cs);
+        }
 #pragma unroll
         for (int s = 0; s < 4; s++) {
-            uint32_t c = present_sample(im, diagram, edge_count, v, X, Y, sox[s], soy[s]);
+            const uint32_t c = cs[s];
 #pragma unroll
             for (int b = 0; b < 4; b++) sum[b] += (c >> (8 * b)) & 0xFFu;
         }

@@ -338,10 +338,18 @@ def main():
             th.join()
         el = time.perf_counter() - t0
         n = sum(counts)
+        # and on one thread, for a shorter sample
+        n1, t1 = 0, time.perf_counter()
+        while time.perf_counter() - t1 < args.cpu_seconds / 3 or n1 == 0:
+            O.depixelize(host[n1 % max_b], W, H, scale=S, subdivide=bool(sub))
+            n1 += 1
+        el1 = time.perf_counter() - t1
         cpu = {"value": round(n / el, 3), "unit": "frames/s", "cores": cores, "kind": "port",
                "sample": f"{n} frames of the same 256x224->4x workload (seeds 0..{max_b - 1}), "
-                         f"oracle on {cores} threads, {el:.1f} s"}
-        log(f"cpu baseline: {n / el:.1f} frames/s on {cores} threads")
+                         f"oracle on {cores} threads, {el:.1f} s",
+               "single_thread": {"value": round(n1 / el1, 3), "unit": "frames/s",
+                                 "sample": f"{n1} frames, oracle on 1 thread, {el1:.1f} s"}}
+        log(f"cpu baseline: {n / el:.1f} frames/s on {cores} threads, {n1 / el1:.1f} on one")
 
     out = {
         "metric": "frames/s batched (256x224->4x) + p99 frame latency vs plain per-frame path",

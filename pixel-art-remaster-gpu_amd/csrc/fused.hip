@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <type_traits>
 #include <vector>
 
 #include "common.hpp"
@@ -469,12 +470,19 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     __shared__ int8_t s_hull[SUB ? RHX * RHY * 17 : 1];
     __shared__ uint16_t s_list[RTX * RTY];  // own cells that need the candidate search
     __shared__ int s_nlist;
-    // resolved pixels of the first PX_CELLS listed cells, so that every own
+    // resolved pixels of the first listed cells, so that every own
     // cell's rows are stored together (a line is then written by one store
-    // instruction instead of being split between the direct and searched cells)
-    constexpr int PX_WORDS = 2048, PX_CELLS = PX_WORDS / (S * S);
+    // instruction instead of being split between the direct and searched
+    // cells).  A pixel is kept as its colour; from S = 6 a tile whose listed
+    // cells would overflow that keeps the code of each pixel's winning
+    // candidate instead (1 byte: (1 - dy) * 3 + (1 - dx), 9 = clear colour),
+    // four times as many cells.  Measured: codes cost a decode per pixel, so
+    // tiles that fit keep colours; at S = 5 codes lose to the overflow stores.
+    constexpr bool CODES = S >= 6;
+    constexpr int PX_WORDS = 2048, PX_CELLS = PX_WORDS / (S * S), PX_CODE_CELLS = 4 * PX_WORDS / (S * S);
     static_assert(PX_CELLS >= 1, "at least one buffered cell");
     __shared__ uint32_t s_px[PX_WORDS];
+    uint8_t* s_px8 = reinterpret_cast<uint8_t*>(s_px);
     __shared__ uint16_t s_idx[RTX * RTY];    // list index of a listed own cell
     __shared__ uint8_t s_simple[RTX * RTY];  // own cell shows its own colour only
     const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
@@ -717,6 +725,10 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     // against the 3x3 candidates in draw order (highest cell index first).
     constexpr uint32_t FULL = (S >= 32) ? 0xFFFFFFFFu : ((1u << S) - 1u);
     const int nunits = s_nlist * S;
+    // uniform over the block; at 6x and 7x only tiles far beyond the colour
+    // buffer gain from codes (measured: sprites lose 5% at a threshold of 1x)
+    const bool codes = CODES && s_nlist > (S >= 8 ? 1 : 3) * PX_CELLS;
+    const int px_cells = codes ? PX_CODE_CELLS : PX_CELLS;
     auto fetch_rows = [&](int u, uint32_t* rows) {
         int own = s_list[u / S], ky = u % S;
         int ux = own % RTX, uy = own / RTX;
@@ -739,6 +751,40 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         int own = s_list[u / S], ky = u % S;
         int ux = own % RTX, uy = own / RTX;
         uint32_t col[S];
+        if constexpr (CODES) {
+            if (codes) {  // the same search, keeping candidate codes
+                uint8_t code[S];
+#pragma unroll
+                for (int q = 0; q < S; q++) code[q] = 9;
+                uint32_t assigned = 0;
+#pragma unroll
+                for (int dy = 1; dy >= -1; dy--) {
+#pragma unroll
+                    for (int dx = 1; dx >= -1; dx--) {
+                        uint32_t row = rows[(1 - dy) * 3 + (1 - dx)];
+                        int sh = -dx * S - KMIN;
+                        uint32_t nb = (sh >= 0 ? (row >> sh) : (row << -sh)) & FULL & ~assigned;
+                        if (nb) {
+#pragma unroll
+                            for (int q = 0; q < S; q++)
+                                if ((nb >> q) & 1u) code[q] = (uint8_t)((1 - dy) * 3 + (1 - dx));
+                            assigned |= nb;
+                        }
+                    }
+                }
+                if (u / S < PX_CODE_CELLS) {
+#pragma unroll
+                    for (int q = 0; q < S; q++) s_px8[u * S + q] = code[q];
+                } else {
+#pragma unroll
+                    for (int q = 0; q < S; q++)
+                        col[q] = code[q] == 9 ? 0xFF000000u
+                                              : s_col[(uy + 2 - code[q] / 3) * RCX + (ux + 2 - code[q] % 3)];
+                    store_row(x0 + ux, (y0 + uy) * S + ky, col);  // beyond the buffer: written directly
+                }
+                return;
+            }
+        }
 #pragma unroll
         for (int q = 0; q < S; q++) col[q] = 0xFF000000u;
         uint32_t assigned = 0;
@@ -783,17 +829,30 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     // instructions), so the tile's output is stored in chunks of CW pixels
     // inside one cell, chunk c by thread c % 256: adjacent lanes store
     // adjacent chunks of a row.  Searched cells beyond the pixel buffer were
-    // stored by the search itself; their chunks are skipped.  Measured: with
-    // chunks of fewer than 4 pixels the chunk loop's own work outweighs the
-    // coalescing once cells overflow (dense 5x frames, 6x sprites), so such
-    // tiles keep the per-cell stores; at 8x overflowing tiles gain 15%.
+    // stored by the search itself; their chunks are skipped.  Measured: at 5x
+    // to 7x, with chunks of fewer than 4 pixels, the chunk loop's own work
+    // outweighs the coalescing once a tile's cells overflow the colour buffer
+    // (dense frames, sprites), so such tiles keep the per-cell stores; at 8x
+    // and from 9x up (rows of 36 bytes and more) chunks win on every tile.
     constexpr int CW = (S % 4 == 0) ? 4 : ((S % 2 == 0) ? 2 : 1);
+    // pixel (kx, ky) of listed cell i, own cell (lx, ly), from the buffer;
+    // use_codes (std::true_type: the tile keeps candidate codes) is fixed per
+    // tile, outside the store loops
+    auto buffered = [&](auto use_codes, int i, int kx, int ky, int lx, int ly) -> uint32_t {
+        if constexpr (decltype(use_codes)::value) {
+            const int code = s_px8[(i * S + ky) * S + kx];
+            return code == 9 ? 0xFF000000u : s_col[(ly + 2 - code / 3) * RCX + (lx + 2 - code % 3)];
+        } else {
+            return s_px[(i * S + ky) * S + kx];
+        }
+    };
     bool chunked = false;
     if constexpr (S != 1 && S != 2 && S != 4) {
         const uintptr_t base = reinterpret_cast<uintptr_t>(out + it.out_off);
-        chunked = ((base | (uintptr_t)it.out_stride) & (4 * CW - 1)) == 0 && (CW == 4 || s_nlist <= PX_CELLS);
+        chunked = ((base | (uintptr_t)it.out_stride) & (4 * CW - 1)) == 0 &&
+                  (CW == 4 || S >= 9 || s_nlist <= PX_CELLS);
     }
-    if (chunked) {
+    auto store_chunks = [&](auto use_codes) {
         constexpr int CPC = S / CW;          // chunks per cell row
         constexpr int CPR = RTX * CPC;       // chunks per output row of the tile
         constexpr int NCH = RTY * S * CPR;   // chunks of the tile
@@ -810,10 +869,9 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 for (int q = 0; q < CW; q++) col[q] = v;
             } else {
                 const int i = s_idx[own];
-                if (i >= PX_CELLS) continue;  // stored by the search
-                const uint32_t* p = s_px + (i * S + ky) * S + kx;
+                if (i >= px_cells) continue;  // stored by the search
 #pragma unroll
-                for (int q = 0; q < CW; q++) col[q] = p[q];
+                for (int q = 0; q < CW; q++) col[q] = buffered(use_codes, i, kx + q, ky, lx, ly);
             }
             const int Y = (y0 + ly) * S + ky;
             const int row = out_topdown(it.flags) ? (OH - 1 - Y) : Y;
@@ -828,6 +886,24 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 __builtin_nontemporal_store(col[0], reinterpret_cast<uint32_t*>(dst));
             }
         }
+    };
+    auto store_cell = [&](auto use_codes, int i) {
+#pragma unroll
+        for (int ky = 0; ky < S; ky++) {
+            uint32_t col[S];
+#pragma unroll
+            for (int q = 0; q < S; q++) col[q] = buffered(use_codes, i, q, ky, cx, cy);
+            store_row(x0 + cx, (y0 + cy) * S + ky, col);
+        }
+    };
+    if (chunked) {
+        if constexpr (CODES) {
+            if (codes) {
+                store_chunks(std::true_type{});
+                return;
+            }
+        }
+        store_chunks(std::false_type{});
         return;
     }
     if (simple) {
@@ -839,14 +915,14 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         for (int ky = 0; ky < S; ky++) store_row(x0 + cx, (y0 + cy) * S + ky, col);
     } else if (mine) {
         const int i = s_idx[t];
-        if (i < PX_CELLS) {
-#pragma unroll
-            for (int ky = 0; ky < S; ky++) {
-                uint32_t col[S];
-#pragma unroll
-                for (int q = 0; q < S; q++) col[q] = s_px[(i * S + ky) * S + q];
-                store_row(x0 + cx, (y0 + cy) * S + ky, col);
+        if (i < px_cells) {
+            if constexpr (CODES) {
+                if (codes) {
+                    store_cell(std::true_type{}, i);
+                    return;
+                }
             }
+            store_cell(std::false_type{}, i);
         }
     }
 }

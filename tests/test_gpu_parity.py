@@ -445,3 +445,25 @@ def test_large_frame_at_8x(gpu, D, O):
     ref = O.depixelize(img, W, H, scale=S)
     assert_equal(graphs[0], ref["graph"], "graph")
     assert_equal(outs[0], ref["rgba"], "rgba")
+
+
+@pytest.mark.parametrize("S", list(range(5, 16)))
+def test_dense_tiles_every_large_scale(gpu, D, O, S):
+    """Tiles where nearly every cell needs the candidate search: a frame of
+    random pixels from three colours (every 32x8-cell tile dense, so the
+    searched pixels overflow a tile's colour buffer and, at 12x and up, its
+    candidate-code buffer too), a dense-crossing frame and a ragged sprite,
+    in both row orders, at scales 5 to 15."""
+    torch = gpu
+    rng = np.random.default_rng(1000 + S)
+    pal = np.array([[20, 30, 200], [240, 240, 240], [90, 160, 40]], dtype=np.uint8)
+    noise = np.ascontiguousarray(pal[rng.integers(0, 3, size=(48, 96))].reshape(48, 96 * 3))
+    frames = [(noise, 96, 48, 3), (D.synth_frame(4, 60 + S, 128, 64, 3), 128, 64, 3),
+              (D.synth_frame(1, 80 + S, 37, 29, 4), 37, 29, 4)]
+    flags = [0, D.TOPDOWN, D.ALPHA | D.TOPDOWN]
+    print("scale", S, flush=True)
+    outs, graphs = run_batch(torch, D, frames, [(S, False, f) for f in flags])
+    for k, ((img, W, H, ch), f) in enumerate(zip(frames, flags)):
+        ref = O.depixelize(img, W, H, scale=S, channels=ch, flags=f)
+        assert_equal(graphs[k], ref["graph"], f"scale {S} item {k} graph")
+        assert_equal(outs[k], ref["rgba"], f"scale {S} item {k} rgba")

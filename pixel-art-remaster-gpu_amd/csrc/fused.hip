@@ -231,6 +231,32 @@ __host__ __device__ inline uint32_t coverage_info(const uint32_t* rows, int S)
     return info | (full ? 1u : 0u);
 }
 
+// Cell masks (scales up to kMaxCellScale): entry j = (1 - dy) * 3 + (1 - dx)
+// of key k holds, as bit ky * S + q, whether a cell of key k covers sample q
+// of row ky of the cell at (-dx, -dy) from it, i.e. what k contributes when
+// it is the candidate at (dx, dy) of that cell.  They follow the row table
+// in the same allocation.
+constexpr int kMaxCellScale = 2;
+#ifndef DPX_RASTER_SUB_TU
+__global__ void kt_cell(const uint32_t* __restrict__ mask, int S, uint16_t* __restrict__ cell)
+{
+    int gid = blockIdx.x * blockDim.x + threadIdx.x;
+    if (gid >= kKeys * 9) return;
+    const int key = gid / 9, j = gid % 9, dy = 1 - j / 3, dx = 1 - j % 3;
+    const int kmin = mask_kmin(S), mw = mask_w(S);
+    const uint32_t full = (1u << S) - 1u;
+    const int sh = -dx * S - kmin;
+    uint32_t m = 0;
+    for (int ky = 0; ky < S; ky++) {
+        const int r = ky - dy * S - kmin;
+        if (r < 0 || r >= mw) continue;
+        const uint32_t row = mask[(size_t)key * mw + r];
+        m |= ((sh >= 0 ? (row >> sh) : (row << -sh)) & full) << (ky * S);
+    }
+    cell[gid] = (uint16_t)m;
+}
+#endif
+
 #ifndef DPX_RASTER_SUB_TU
 __global__ void kt_info(const uint32_t* __restrict__ mask, int S, uint16_t* __restrict__ info)
 {
@@ -813,7 +839,46 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         }
     };
 
-    if (nunits > 0) {  // uniform: read from LDS after the barrier
+    if constexpr (!SUB && S <= kMaxCellScale) {
+        // 1x and 2x: one thread per listed cell, the candidates' samples of
+        // it from the cell-mask table (nine independent loads), resolved in
+        // the same draw order as the rows of the other scales.  Measured: 2x
+        // frames 8-11% faster than with row units; at 4x a thread resolving
+        // 16 pixels alone lengthens sparse tiles (-3.5% on 16-colour frames).
+        const uint16_t* cell_table = reinterpret_cast<const uint16_t*>(mask_table + kKeys * MW);
+        const int nlist = s_nlist;
+        for (int i = t; i < nlist; i += 256) {
+            const int own = s_list[i];
+            const int ux = own % RTX, uy = own / RTX;
+            uint32_t m[9], c[9];
+#pragma unroll
+            for (int j = 0; j < 9; j++) {
+                const int k = (uy + 2 - j / 3) * RCX + (ux + 2 - j % 3);
+                const uint32_t key = s_key[k];
+                m[j] = key != kNoCell ? cell_table[key * 9 + j] : 0u;
+                c[j] = s_col[k];
+            }
+            uint32_t col[S * S];
+#pragma unroll
+            for (int p = 0; p < S * S; p++) col[p] = 0xFF000000u;
+            uint32_t assigned = 0;
+#pragma unroll
+            for (int j = 0; j < 9; j++) {
+                const uint32_t nb = m[j] & ~assigned;
+#pragma unroll
+                for (int p = 0; p < S * S; p++) col[p] = ((nb >> p) & 1u) ? c[j] : col[p];
+                assigned |= nb;
+            }
+            if (i < PX_CELLS) {
+#pragma unroll
+                for (int p = 0; p < S * S; p++) s_px[i * S * S + p] = col[p];
+            } else {
+#pragma unroll
+                for (int ky = 0; ky < S; ky++) store_row(x0 + ux, (y0 + uy) * S + ky, col + ky * S);
+            }
+        }
+        __syncthreads();
+    } else if (nunits > 0) {  // uniform: read from LDS after the barrier
         for (int u = t; u < nunits; u += 256) {
             uint32_t rows[9];
             fetch_rows(u, rows);
@@ -1024,9 +1089,13 @@ int ensure_mask(dpx_context* ctx, int S)
 {
     if (ctx->d_mask[S] && ctx->d_info[S]) return DPX_OK;
     size_t n = (size_t)kKeys * mask_w(S);
-    if (!ctx->d_mask[S]) DPX_HIP(hipMalloc(&ctx->d_mask[S], n * sizeof(uint32_t)));
+    const size_t cell_bytes = S <= kMaxCellScale ? (size_t)kKeys * 9 * sizeof(uint16_t) : 0;
+    if (!ctx->d_mask[S]) DPX_HIP(hipMalloc(&ctx->d_mask[S], n * sizeof(uint32_t) + cell_bytes));
     if (!ctx->d_info[S]) DPX_HIP(hipMalloc(&ctx->d_info[S], kKeys * sizeof(uint16_t)));
     hipLaunchKernelGGL(kt_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, ctx->d_hull, S, ctx->d_mask[S]);
+    if (cell_bytes)
+        hipLaunchKernelGGL(kt_cell, dim3((unsigned)((kKeys * 9 + 255) / 256)), dim3(256), 0, 0, ctx->d_mask[S], S,
+                           reinterpret_cast<uint16_t*>(ctx->d_mask[S] + n));
     hipLaunchKernelGGL(kt_info, dim3(kKeys / 256), dim3(256), 0, 0, ctx->d_mask[S], S, ctx->d_info[S]);
     DPX_HIP(hipGetLastError());
     DPX_HIP(hipDeviceSynchronize());

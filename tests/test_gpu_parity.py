@@ -447,13 +447,13 @@ def test_large_frame_at_8x(gpu, D, O):
     assert_equal(outs[0], ref["rgba"], "rgba")
 
 
-@pytest.mark.parametrize("S", list(range(5, 16)))
-def test_dense_tiles_every_large_scale(gpu, D, O, S):
+@pytest.mark.parametrize("S", list(range(1, 16)))
+def test_dense_tiles_every_scale(gpu, D, O, S):
     """Tiles where nearly every cell needs the candidate search: a frame of
     random pixels from three colours (every 32x8-cell tile dense, so the
     searched pixels overflow a tile's colour buffer and, at 12x and up, its
     candidate-code buffer too), a dense-crossing frame and a ragged sprite,
-    in both row orders, at scales 5 to 15."""
+    in both row orders, at every scale 1 to 15."""
     torch = gpu
     rng = np.random.default_rng(1000 + S)
     pal = np.array([[20, 30, 200], [240, 240, 240], [90, 160, 40]], dtype=np.uint8)

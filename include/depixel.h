@@ -214,7 +214,11 @@ void dpx_context_destroy(dpx_context* ctx);
 int dpx_context_reserve(dpx_context* ctx, int max_items, int64_t max_item_pixels, int scale);
 /* Enqueue the batch on `stream`; returns once enqueued.  The context's
  * scratch (graph buffers, descriptors) is reused across calls: calls on one
- * context must be serialised by the caller (same stream or explicit sync). */
+ * context must be serialised by the caller (same stream or explicit sync).
+ * A call whose items (and output alignment mod 16) equal the previous call's
+ * only launches kernels on `stream`, so it may be captured into a HIP graph
+ * (after one ordinary call with those items, and with timing off); the
+ * graph reads d_in and writes d_out at replay time. */
 int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_out, const dpx_item* items,
                   int n_items, void* stream);
 /* Whether the batches that follow also write the resolved graph of every

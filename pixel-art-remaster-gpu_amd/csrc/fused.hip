@@ -952,12 +952,20 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             }
         }
     };
+    // A thread's own cell, simple or buffered, through one sequence of store
+    // instructions for the whole wave: with separate branches for the two
+    // kinds, a wave holding both issues every row store twice, each writing
+    // part of the same lines.
     auto store_cell = [&](auto use_codes, int i) {
+        const uint32_t c = s_col[kown];
 #pragma unroll
         for (int ky = 0; ky < S; ky++) {
             uint32_t col[S];
 #pragma unroll
-            for (int q = 0; q < S; q++) col[q] = buffered(use_codes, i, q, ky, cx, cy);
+            for (int q = 0; q < S; q++) {
+                const uint32_t b = buffered(use_codes, i, q, ky, cx, cy);
+                col[q] = simple ? c : b;
+            }
             store_row(x0 + cx, (y0 + cy) * S + ky, col);
         }
     };
@@ -971,16 +979,9 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         store_chunks(std::false_type{});
         return;
     }
-    if (simple) {
-        uint32_t col[S];
-        uint32_t c = s_col[kown];
-#pragma unroll
-        for (int q = 0; q < S; q++) col[q] = c;
-#pragma unroll
-        for (int ky = 0; ky < S; ky++) store_row(x0 + cx, (y0 + cy) * S + ky, col);
-    } else if (mine) {
-        const int i = s_idx[t];
-        if (i < px_cells) {
+    if (mine) {
+        const int i = simple ? 0 : s_idx[t];  // a simple cell reads entry 0 and ignores it
+        if (simple || i < px_cells) {         // listed cells beyond the buffer: stored by the search
             if constexpr (CODES) {
                 if (codes) {
                     store_cell(std::true_type{}, i);

@@ -452,10 +452,26 @@ __global__ void k_cc_union(const uint8_t* __restrict__ graph, int W, int H, int3
     }
 }
 
+// The root of v without shortening paths: the final pass writes each
+// label over its own parent entry (the arrays alias), so it must write no
+// other entry -- a halving write could replace a label another thread has
+// already set to its root with a non-root ancestor.  Entries it reads are
+// then an old parent or the root, both ancestors.
+__device__ inline int uf_root(const int32_t* parent_, int v)
+{
+    const volatile int32_t* parent = parent_;
+    int p = parent[v];
+    while (p != v) {
+        v = p;
+        p = parent[v];
+    }
+    return v;
+}
+
 __global__ void k_cc_final(int32_t* parent, int32_t* labels, int N)
 {
     int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < N) labels[k] = uf_find(parent, k);
+    if (k < N) labels[k] = uf_root(parent, k);
 }
 
 // Border chains (extractBorderPoints cc_functions.cu:348-503) of one frame,

@@ -356,6 +356,25 @@ def gpu_border_chains(torch, D, g, max_nodes=None, max_chains=None):
     return out, nn, status
 
 
+def test_component_labels_repeat_on_larger_graphs(gpu, D, O):
+    """Union-find labels (canonical: the smallest index of the component) on
+    resolved graphs of whole frames, each run three times: the final pass
+    must not disturb labels other threads have already written."""
+    torch = gpu
+    s = torch.cuda.current_stream()
+    for kind, seed, W, H in ((0, 41, 256, 224), (2, 42, 320, 224), (4, 43, 200, 150), (2, 44, 1, 57)):
+        img = D.synth_frame(kind, seed, W, H, 3)
+        g = O.heuristic_cross(O.trivial_cross(O.similarity_graph(img, W, H)))
+        ref = O.components(g)
+        d_g = torch.from_numpy(np.ascontiguousarray(g).reshape(-1)).to("cuda:0")
+        for rep in range(3):
+            print("components", kind, seed, W, H, "run", rep, flush=True)
+            d_lab = torch.full((W * H,), -1, dtype=torch.int32, device="cuda:0")
+            D.check(D.lib.dpx_stage_components(d_g.data_ptr(), W, H, d_lab.data_ptr(), s.cuda_stream))
+            torch.cuda.synchronize()
+            assert_equal(d_lab.cpu().numpy().reshape(H, W), ref, f"labels {kind}/{seed} run {rep}")
+
+
 def test_border_chains_match_oracle(gpu, D, O):
     torch = gpu
     import json

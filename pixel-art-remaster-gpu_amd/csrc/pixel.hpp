@@ -62,9 +62,9 @@ __host__ __device__ inline Px load_px(const ImgView& im, int i, int j)
 // integer, so the truncated double equals n / 1000.  Only for the exact
 // multiples (16,774 of the 2^24 colours, every grey among them) can the
 // double land just below the integer; those take the double expression.
-// The quotient comes from single precision: n * 0.001f + 0.0005f is within
-// 5e-5 of n/1000 + 0.0005, and n/1000 + 0.0005 is at least 0.0005 from an
-// integer for every n, so it truncates to floor(n / 1000).
+// The quotient is a high product: with m = ceil(2^32 / 1000) = 4294968,
+// (n * m) >> 32 = floor(n / 1000) while n * (1000 m - 2^32) = 704 n < 2^32,
+// i.e. for every n up to 6.1 M (n <= 255,000 here; checked exhaustively).
 __host__ __device__ inline uint32_t mul24(uint32_t a, uint32_t b)  // a, b < 2^24
 {
 #ifdef __HIP_DEVICE_COMPILE__
@@ -78,7 +78,7 @@ __host__ __device__ inline int luma(int c0, int c1, int c2)
 {
 #pragma clang fp contract(off)
     int n = 299 * c0 + 587 * c1 + 114 * c2;
-    int q = (int)((float)n * 0.001f + 0.0005f);
+    int q = (int)(uint32_t)(((uint64_t)(uint32_t)n * 4294968u) >> 32);
     if ((int)mul24((uint32_t)q, 1000u) != n) return q;
     double yd = 0.299 * (double)c0;
     yd = yd + 0.587 * (double)c1;

@@ -911,12 +911,60 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             return s_px[(i * S + ky) * S + kx];
         }
     };
-    bool chunked = false;
+    bool chunked = false, wide = false;
     if constexpr (S != 1 && S != 2 && S != 4) {
         const uintptr_t base = reinterpret_cast<uintptr_t>(out + it.out_off);
         chunked = ((base | (uintptr_t)it.out_stride) & (4 * CW - 1)) == 0 &&
                   (CW == 4 || S >= 9 || s_nlist <= PX_CELLS);
+        // odd scales: chunks of 4 pixels across cell boundaries when the rows
+        // are 16-byte aligned and every listed cell is buffered (no pixel of
+        // the tile was stored by the search)
+        if constexpr (CW == 1) wide = ((base | (uintptr_t)it.out_stride) & 15) == 0 && s_nlist <= px_cells;
     }
+    // Chunks of 4 pixels of a tile row (16-byte stores) that may span two
+    // cells (S >= 3): each pixel takes its colour from its own cell, simple
+    // or buffered.  Measured against one-pixel chunks: 3x 9-10%, 5x 15%, 7x
+    // 8% faster; against the two-pixel chunks of 6x, 8-12% slower (the
+    // per-pixel lookups cost more than the wider stores save).
+    auto store_wide = [&](auto use_codes) {
+        constexpr int CPR = RTX * S / 4;   // chunks per output row of the tile
+        constexpr int NCH = RTY * S * CPR;
+        static_assert(RTX * S % 4 == 0, "whole chunks per tile row");
+        const int ncx = min(RTX, it.W - x0), rowpx = ncx * S;  // cells and pixels of a tile row in the frame
+        for (int c = t; c < NCH; c += 256) {
+            const int ry = c / CPR, px0 = 4 * (c % CPR);
+            const int ly = ry / S, ky = ry % S;
+            if (px0 >= rowpx || y0 + ly >= it.H) continue;
+            // the chunk's cells; past the frame's last column the pixels are
+            // not stored and take that column's cell
+            const int la = px0 / S, lb = min((px0 + 3) / S, ncx - 1);
+            const int oa = ly * RTX + la, ob = ly * RTX + lb;
+            const bool sa = s_simple[oa], sb = s_simple[ob];
+            const int ia = sa ? 0 : s_idx[oa], ib = sb ? 0 : s_idx[ob];  // simple: entry 0, ignored
+            const uint32_t va = s_col[(ly + 1) * RCX + la + 1], vb = s_col[(ly + 1) * RCX + lb + 1];
+            uint32_t col[4];
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int p = px0 + q;
+                const bool inb = p >= (la + 1) * S;  // pixel in the second cell
+                const int lx = inb ? lb : la, i = inb ? ib : ia;
+                const int kx = min(p - lx * S, S - 1);  // (past the frame: not stored)
+                const uint32_t bv = buffered(use_codes, i, kx, ky, lx, ly);
+                col[q] = (inb ? sb : sa) ? (inb ? vb : va) : bv;
+            }
+            const int Y = (y0 + ly) * S + ky;
+            const int row = out_topdown(it.flags) ? (OH - 1 - Y) : Y;
+            uint8_t* dst = out + it.out_off + (size_t)row * it.out_stride + ((size_t)x0 * S + px0) * 4;
+            if (px0 + 3 < rowpx) {
+                u32x4 v = { col[0], col[1], col[2], col[3] };
+                __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(dst));
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++)
+                    if (px0 + q < rowpx) __builtin_nontemporal_store(col[q], reinterpret_cast<uint32_t*>(dst + 4 * q));
+            }
+        }
+    };
     auto store_chunks = [&](auto use_codes) {
         constexpr int CPC = S / CW;          // chunks per cell row
         constexpr int CPR = RTX * CPC;       // chunks per output row of the tile
@@ -969,6 +1017,16 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             store_row(x0 + cx, (y0 + cy) * S + ky, col);
         }
     };
+    if (wide) {
+        if constexpr (CODES) {
+            if (codes) {
+                store_wide(std::true_type{});
+                return;
+            }
+        }
+        store_wide(std::false_type{});
+        return;
+    }
     if (chunked) {
         if constexpr (CODES) {
             if (codes) {

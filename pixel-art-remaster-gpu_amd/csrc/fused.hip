@@ -758,6 +758,10 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     auto fetch_rows = [&](int u, uint32_t* rows) {
         int own = s_list[u / S], ky = u % S;
         int ux = own % RTX, uy = own / RTX;
+        // an own cell that covers its whole square takes every pixel the
+        // cells drawn after it leave: the candidates drawn before it (left,
+        // down-left, down, down-right) can win none, and are not fetched
+        const bool own_full = s_info[(uy + 1) * RCX + ux + 1] & 1u;
 #pragma unroll
         for (int dy = 1; dy >= -1; dy--) {
 #pragma unroll
@@ -765,7 +769,8 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 int k = (uy + 1 + dy) * RCX + (ux + 1 + dx);
                 int r = ky - dy * S - KMIN;
                 uint32_t row = 0;
-                if (r >= 0 && r < MW && s_key[k] != kNoCell) {
+                const bool before_own = dy < 0 || (dy == 0 && dx < 0);
+                if (!(own_full && before_own) && r >= 0 && r < MW && s_key[k] != kNoCell) {
                     if constexpr (SUB) row = s_mask[k * MW + r];
                     else row = mask_table[(uint32_t)s_key[k] * MW + r];
                 }

@@ -928,9 +928,10 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     }
     // Chunks of 4 pixels of a tile row (16-byte stores) that may span two
     // cells (S >= 3): each pixel takes its colour from its own cell, simple
-    // or buffered.  Measured against one-pixel chunks: 3x 9-10%, 5x 15%, 7x
-    // 8% faster; against the two-pixel chunks of 6x, 8-12% slower (the
-    // per-pixel lookups cost more than the wider stores save).
+    // or buffered.  Measured against one-pixel chunks: 3x 14% and 5x 24%
+    // faster (7x: 8% before the path for two simple cells); against the
+    // two-pixel chunks of 6x, 6-12% slower (the per-pixel lookups cost more
+    // than the wider stores save).
     auto store_wide = [&](auto use_codes) {
         constexpr int CPR = RTX * S / 4;   // chunks per output row of the tile
         constexpr int NCH = RTY * S * CPR;
@@ -948,14 +949,19 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             const int ia = sa ? 0 : s_idx[oa], ib = sb ? 0 : s_idx[ob];  // simple: entry 0, ignored
             const uint32_t va = s_col[(ly + 1) * RCX + la + 1], vb = s_col[(ly + 1) * RCX + lb + 1];
             uint32_t col[4];
+            if (sa && sb) {  // both cells simple (most chunks): no buffer reads
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
-                const int p = px0 + q;
-                const bool inb = p >= (la + 1) * S;  // pixel in the second cell
-                const int lx = inb ? lb : la, i = inb ? ib : ia;
-                const int kx = min(p - lx * S, S - 1);  // (past the frame: not stored)
-                const uint32_t bv = buffered(use_codes, i, kx, ky, lx, ly);
-                col[q] = (inb ? sb : sa) ? (inb ? vb : va) : bv;
+                for (int q = 0; q < 4; q++) col[q] = px0 + q >= (la + 1) * S ? vb : va;
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    const int p = px0 + q;
+                    const bool inb = p >= (la + 1) * S;  // pixel in the second cell
+                    const int lx = inb ? lb : la, i = inb ? ib : ia;
+                    const int kx = min(p - lx * S, S - 1);  // (past the frame: not stored)
+                    const uint32_t bv = buffered(use_codes, i, kx, ky, lx, ly);
+                    col[q] = (inb ? sb : sa) ? (inb ? vb : va) : bv;
+                }
             }
             const int Y = (y0 + ly) * S + ky;
             const int row = out_topdown(it.flags) ? (OH - 1 - Y) : Y;

@@ -49,13 +49,14 @@ def bench_batch_config(args, torch, D):
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
     spec = config_items(D, args.config)
+    sub = 1 if args.subdivide else 0
     hosts, items = [], []
     in_off = out_off = 0
     algo = 0
     for kind, seed, w, h, ch, flags, s in spec:
         img = D.synth_frame(kind, seed, w, h, ch)
         hosts.append(img)
-        items.append(D.Item(in_off, out_off, w, h, img.shape[1], ch, s, 0, flags, w * s * 4))
+        items.append(D.Item(in_off, out_off, w, h, img.shape[1], ch, s, sub, flags, w * s * 4))
         algo += w * h * ch + (w * s) * (h * s) * 4
         in_off += img.size
         out_off += (w * s) * (h * s) * 4
@@ -85,15 +86,18 @@ def bench_batch_config(args, torch, D):
         it = items[k]
         n = it.out_stride * it.height * it.scale
         got = d_out[it.out_offset:it.out_offset + n].cpu().numpy()
-        ref, _, _ = D.frame(hosts[k], it.width, it.height, it.channels, scale=it.scale, flags=it.flags)
+        ref, _, _ = D.frame(hosts[k], it.width, it.height, it.channels, scale=it.scale, subdivide=bool(sub),
+                            flags=it.flags)
         ok = ok and bool(np.array_equal(got, ref.reshape(-1)))
     ctx.close()
     n_items = len(items)
     out = {
-        "metric": f"items/s batched ({args.config})", "value": round(n_items * steps / wall, 1), "unit": "items/s",
+        "metric": f"items/s batched ({args.config}{', subdivided' if sub else ''})",
+        "value": round(n_items * steps / wall, 1), "unit": "items/s",
         "n_gpus": 1, "steps": steps, "ms_per_step": round(wall * 1e3 / steps, 4), "higher_is_better": True,
         "dtype": "u8", "data": "synthetic",
-        "config": {"workload": args.config, "items_per_step": n_items, "algorithmic_bytes_per_step": algo},
+        "config": {"workload": args.config, "subdivide": bool(sub), "items_per_step": n_items,
+                   "algorithmic_bytes_per_step": algo},
         "kernel_ms": {"graph": round(graph_ms, 5), "raster": round(raster_ms, 5)},
         "roofline": {"bound": "hbm", "achieved": round(algo / (raster_ms * 1e-3) / 1e9, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(algo / (raster_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
@@ -205,7 +209,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a plain 1 GiB device copy (reference for HBM byte counters)")
     ap.add_argument("--subdivide", action="store_true",
-                    help="c0 with corner-cutting subdivision on (the reference's 's' key)")
+                    help="corner-cutting subdivision on (the reference's 's' key): c0, c1, c2, c4")
     ap.add_argument("--config", default="c0", choices=["c0", "c1", "c2", "c3", "c4"],
                     help="c0 (default): the headline 256x224 -> 4x line; c1 4096 alpha sprites -> 6x; "
                          "c2 64 frames of five console sizes -> 4x; c3 128 served 320x224 streams at 60 Hz; "

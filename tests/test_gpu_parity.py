@@ -466,6 +466,21 @@ def test_large_frame_at_8x(gpu, D, O):
     assert_equal(outs[0], ref["rgba"], "rgba")
 
 
+def test_large_frames_at_8x_subdivided(gpu, D, O):
+    """Two 640x480 dense-crossing frames at 8x with spline subdivision in one
+    batch (5120x3840 each): graphs and pixels equal the oracle's, whose
+    subdivision is exact lattice arithmetic (no per-vertex error to bound)."""
+    torch = gpu
+    W, H, S = 640, 480, 8
+    frames = [(D.synth_frame(4, 70 + k, W, H, 3, W * 3), W, H, 3) for k in range(2)]
+    outs, graphs = run_batch(torch, D, frames, [(S, True, 0), (S, True, D.TOPDOWN)])
+    for k, flags in enumerate((0, D.TOPDOWN)):
+        print("frame", k, flush=True)
+        ref = O.depixelize(frames[k][0], W, H, scale=S, subdivide=True, flags=flags)
+        assert_equal(graphs[k], ref["graph"], f"frame {k} graph")
+        assert_equal(outs[k], ref["rgba"], f"frame {k} rgba")
+
+
 @pytest.mark.parametrize("S", list(range(1, 16)))
 def test_dense_tiles_every_scale(gpu, D, O, S):
     """Tiles where nearly every cell needs the candidate search: a frame of

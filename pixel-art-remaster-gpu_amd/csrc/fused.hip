@@ -157,7 +157,9 @@ __device__ inline int ear_clip_compact(const int* x, const int* y, int n, uint8_
 
 // All mask_w(S) rows of coverage_row at once, with each triangle's vertices
 // read and oriented once (the polygon arrays live in memory, not registers,
-// so coverage_row's per-row re-reads dominate).  Same bits as coverage_row.
+// so coverage_row's per-row re-reads dominate), each row's covered columns
+// from the edges' intervals (tri_row_bits) rather than a test per sample.
+// Same bits as coverage_row.
 template <int S>
 __device__ inline void coverage_rows(const int* x, const int* y, const uint8_t* tri, int nt, uint32_t* rows)
 {
@@ -179,14 +181,7 @@ __device__ inline void coverage_rows(const int* x, const int* y, const uint8_t* 
         for (int r = 0; r < mw; r++) {
             const int sy = (2 * (kmin + r) + 1) * 64;
             if (sy < ymin || sy > ymax) continue;
-            uint32_t bits = 0;
-            for (int b = 0; b < mw; b++) {
-                const int sx = (2 * (kmin + b) + 1) * 64;
-                if (edge_in(ax, ay, bx, by, sx, sy) && edge_in(bx, by, cx, cy, sx, sy) &&
-                    edge_in(cx, cy, ax, ay, sx, sy))
-                    bits |= 1u << b;
-            }
-            rows[r] |= bits;
+            rows[r] |= tri_row_bits(ax, ay, bx, by, cx, cy, sy, kmin, mw);  // the edge_in bits of the row
         }
     }
 }

@@ -379,4 +379,43 @@ __host__ __device__ inline bool tri_covers(T ax, T ay, T bx, T by, T cx, T cy, T
            edge_in(cx, cy, ax, ay, sx, sy);
 }
 
+// floor(a / d) for d > 0 and |a| < 2^26: the single-precision quotient is
+// within 1 of it, and one exact step on each side fixes it.
+__host__ __device__ inline int floor_div(int a, int d)
+{
+    int q = (int)floorf((float)a / (float)d);
+    if (q * d > a) q--;
+    else if ((q + 1) * d <= a) q++;
+    return q;
+}
+
+// Bits b (0 <= b < mw <= 32) of the samples sx = 128 (kmin + b) + 64 on row
+// sy covered by the counter-clockwise triangle (a, b, c), under edge_in's
+// rule -- the same bits as edge_in on every sample.  Along a row an edge's
+// function is e = K - dy * sx: with dy > 0 the covered columns are those
+// with e > 0, b <= floor((N - 1) / D); with dy < 0 (ties cover) those with
+// e >= 0, b >= -floor(N / -D); with dy = 0 all or none; N = e at b = 0, D =
+// 128 dy.  Coordinates are bounded so that every product fits in int.
+__host__ __device__ inline uint32_t tri_row_bits(int ax, int ay, int bx, int by, int cx, int cy, int sy, int kmin,
+                                                 int mw)
+{
+    int lo = 0, hi = mw - 1;
+    auto clip = [&](int px, int py, int qx, int qy) {
+        const int dx = qx - px, dy = qy - py;
+        const int K = dx * (sy - py) + dy * px;  // e at sx = 0
+        if (dy == 0) {
+            if (!(K > 0 || (K == 0 && dx > 0))) hi = -1;
+            return;
+        }
+        const int N = K - dy * (128 * kmin + 64), D = 128 * dy;
+        if (dy > 0) hi = min(hi, floor_div(N - 1, D));
+        else lo = max(lo, -floor_div(N, -D));
+    };
+    clip(ax, ay, bx, by);
+    clip(bx, by, cx, cy);
+    clip(cx, cy, ax, ay);
+    if (lo > hi) return 0u;
+    return ((2u << hi) - 1u) & ~((1u << lo) - 1u);
+}
+
 }  // namespace dpx

@@ -466,6 +466,39 @@ def test_large_frame_at_8x(gpu, D, O):
     assert_equal(outs[0], ref["rgba"], "rgba")
 
 
+def test_full_sprite_batch_at_6x(gpu, D, O):
+    """The 4096-sprite batch at its real size: sprites of six sizes with alpha
+    backgrounds, one launch at 6x; every sprite's graph and pixels equal the
+    oracle's."""
+    torch = gpu
+    frames, spec = [], []
+    for k in range(4096):
+        w, h = D.sprite_size(0, k)
+        frames.append((D.synth_frame(1, k, w, h, 4), w, h, 4))
+        spec.append((6, False, D.ALPHA))
+    print("batch of", len(frames), "sprites", flush=True)
+    outs, graphs = run_batch(torch, D, frames, spec)
+    for k, (img, w, h, ch) in enumerate(frames):
+        ref = O.depixelize(img, w, h, scale=6, channels=4, flags=D.ALPHA)
+        assert_equal(graphs[k], ref["graph"], f"sprite {k} ({w}x{h}) graph")
+        assert_equal(outs[k], ref["rgba"], f"sprite {k} ({w}x{h}) rgba")
+
+
+def test_full_console_batch_at_4x(gpu, D, O):
+    """The 64-frame batch of five console sizes (one-pixel curves, 16-64
+    colours) at its real size, one launch at 4x; every frame equals the
+    oracle."""
+    torch = gpu
+    sizes = [(256, 240)] * 13 + [(256, 224)] * 13 + [(320, 224)] * 13 + [(384, 224)] * 13 + [(240, 160)] * 12
+    frames = [(D.synth_frame(2, k, w, h, 3), w, h, 3) for k, (w, h) in enumerate(sizes)]
+    print("batch of", len(frames), "frames", flush=True)
+    outs, graphs = run_batch(torch, D, frames, [(4, False, 0)] * len(frames))
+    for k, (img, w, h, ch) in enumerate(frames):
+        ref = O.depixelize(img, w, h, scale=4)
+        assert_equal(graphs[k], ref["graph"], f"frame {k} ({w}x{h}) graph")
+        assert_equal(outs[k], ref["rgba"], f"frame {k} ({w}x{h}) rgba")
+
+
 def test_large_frames_at_8x_subdivided(gpu, D, O):
     """Two 640x480 dense-crossing frames at 8x with spline subdivision in one
     batch (5120x3840 each): graphs and pixels equal the oracle's, whose

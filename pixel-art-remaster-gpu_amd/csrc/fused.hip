@@ -9,16 +9,19 @@
 //              snapshot (crossed blocks listed and decided once), cell keys
 //              (and, with subdivision, hulls, smoothed polygons and their
 //              coverage) in LDS; cells whose S*S pixels are all their own
-//              colour stream them to HBM at once, the rest are listed and
-//              resolved row by row against the 3x3 candidate cells in draw
-//              order.
+//              colour are simple, the rest are listed and resolved row by
+//              row (1x, 2x: cell by cell) against the 3x3 candidate cells
+//              in draw order into an LDS pixel buffer; then every row of
+//              the tile's output is stored once, simple and resolved pixels
+//              by the same store instructions.
 //
 // Coverage of an unsmoothed cell depends only on its 12-bit key and the
 // scale, so it is precomputed once per scale into a table of sample-row
 // bitmasks (kt_mask): bit b of row r of key k says whether sample
 // (kmin+b, kmin+r) of the cell's local sample grid is covered, kmin =
-// -(S/2)-1, MW = 2S+2 rows and columns (local [-0.5, 1.5] of the cell); and
-// into a 16-bit summary per key (kt_info, coverage_info).
+// -(S/2)-1, MW = 2S+2 rows and columns (local [-0.5, 1.5] of the cell); into
+// a 16-bit summary per key (kt_info, coverage_info); and at 1x and 2x into
+// each key's sample mask of each neighbour cell (kt_cell).
 //
 // This file is compiled twice.  Built with DPX_RASTER_SUB_TU it yields only
 // the raster kernels with subdivision (pick_raster_sub): their per-thread

@@ -159,7 +159,8 @@ void sprite(Canvas& cv, SplitMix64& r)
     // interior detail: rectangles and dither, clipped to the body
     int nr = r.uni(2, 6);
     for (int k = 0; k < nr; k++) {
-        int w = r.uni(2, std::max(2, cv.W / 2)), h = r.uni(2, std::max(2, cv.H / 2));
+        // (at most the sprite's size: a sprite one pixel wide or tall)
+        int w = std::min(cv.W, r.uni(2, std::max(2, cv.W / 2))), h = std::min(cv.H, r.uni(2, std::max(2, cv.H / 2)));
         int x0 = r.uni(0, cv.W - w), y0 = r.uni(0, cv.H - h);
         const uint8_t* c = &pal[(size_t)r.uni(0, npal - 1) * 4];
         const uint8_t* c2 = &pal[(size_t)r.uni(0, npal - 1) * 4];

@@ -47,6 +47,20 @@ def test_synthetic_generator_is_deterministic(D):
     assert 0.15 < (alpha == 0).mean() < 0.45
 
 
+def test_synthetic_generator_takes_one_pixel_sides(D):
+    """Every kind at sizes with a side of 1 to 3 pixels (the sprite's detail
+    rectangles are clipped to the sprite; a 1-pixel side once divided by
+    zero), written inside the frame only."""
+    import numpy as np
+    for kind in (0, 1, 2, 4):
+        ch = 4 if kind == 1 else 3
+        for W, H in ((1, 1), (1, 5), (5, 1), (3, 1), (1, 3), (148, 1), (1, 120), (2, 2), (3, 3)):
+            stride = W * ch + 3
+            f = D.synth_frame(kind, 11, W, H, ch, stride)
+            assert f.shape == (H, stride)
+            assert not f[:, W * ch:].any(), f"kind {kind} {W}x{H} wrote past the row"
+
+
 def test_argument_checks_need_no_gpu(D):
     import numpy as np
     img = np.zeros((4, 12), dtype=np.uint8)

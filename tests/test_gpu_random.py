@@ -1,4 +1,5 @@
-"""Randomized parity: batches of random items against the oracle.
+"""Randomized parity: batches of random items, and random frames through the
+per-frame stages, against the oracle.
 
 Each batch mixes 24 items of random kind (16-colour art, alpha sprites,
 curves, dense crossings), size (1..160 x 1..120), scale (1..15),
@@ -73,3 +74,39 @@ def test_random_batch_matches_oracle(gpu, D, O, seed):
         bad = np.argwhere((got != ref).any(-1))
         assert len(bad) == 0, f"{what}: {len(bad)} pixels differ, first at {bad[:3].tolist()}"
     assert (out[~covered] == SENTINEL).all(), "bytes outside the images were written"
+
+
+CS = 45
+
+
+@pytest.mark.parametrize("seed", list(range(80)))
+def test_random_frame_stages_match_oracle(gpu, D, O, seed):
+    """The plain per-frame path on a random frame (kind, size 1..120 x 1..90,
+    scale 1..8, subdivision, byte and row order): pixels, graph, edge counts
+    and the defined slots of the triangle diagram equal the oracle's, and so
+    do the component labels of its graph."""
+    torch = gpu
+    rng = np.random.default_rng(9000 + seed)
+    kind = int(rng.choice([0, 1, 2, 4]))
+    ch = 4 if kind == 1 else 3
+    W, H = int(rng.integers(1, 121)), int(rng.integers(1, 91))
+    S, sub = int(rng.integers(1, 9)), bool(rng.random() < 0.4)
+    flags = (D.TOPDOWN if rng.random() < 0.4 else 0) | (D.RGB if rng.random() < 0.3 else 0) | (D.ALPHA if kind == 1 else 0)
+    img = D.synth_frame(kind, 9000 + seed, W, H, ch, W * ch + int(rng.integers(0, 8)))
+    what = f"case {seed}: kind {kind} {W}x{H} ch {ch} x{S} sub {sub} flags {flags}"
+    print(what, flush=True)
+    ref = O.depixelize(img, W, H, scale=S, subdivide=sub, channels=ch, flags=flags, want_diagram=True)
+    rgba, g, ec, diag = D.frame(img, W, H, ch, scale=S, subdivide=sub, flags=flags, want_diagram=True)
+    assert np.array_equal(g, ref["graph"]), what + ": graph"
+    assert np.array_equal(ec, ref["edge_count"]), what + ": edge counts"
+    dd, rd = diag.reshape(-1, CS, 2), ref["diagram"].reshape(-1, CS, 2)
+    for k, m in enumerate(np.asarray(ec).reshape(-1)):
+        w = max(0, (int(m) - 2) * 3)
+        assert np.array_equal(dd[k, :w], rd[k, :w]), f"{what}: triangles of cell {k}"
+    assert np.array_equal(rgba, ref["rgba"]), what + ": pixels"
+    s = torch.cuda.current_stream()
+    d_g = torch.from_numpy(np.ascontiguousarray(ref["graph"]).reshape(-1)).to("cuda:0")
+    d_lab = torch.full((W * H,), -1, dtype=torch.int32, device="cuda:0")
+    D.check(D.lib.dpx_stage_components(d_g.data_ptr(), W, H, d_lab.data_ptr(), s.cuda_stream))
+    torch.cuda.synchronize()
+    assert np.array_equal(d_lab.cpu().numpy().reshape(H, W), O.components(ref["graph"])), what + ": labels"

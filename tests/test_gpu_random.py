@@ -1,5 +1,5 @@
-"""Randomized parity: batches of random items, and random frames through the
-per-frame stages, against the oracle.
+"""Randomized parity against the oracle: batches of random items, random
+frames through the per-frame stages, and random viewer windows.
 
 Each batch mixes 24 items of random kind (16-colour art, alpha sprites,
 curves, dense crossings), size (1..160 x 1..120), scale (1..15),
@@ -110,3 +110,41 @@ def test_random_frame_stages_match_oracle(gpu, D, O, seed):
     D.check(D.lib.dpx_stage_components(d_g.data_ptr(), W, H, d_lab.data_ptr(), s.cuda_stream))
     torch.cuda.synchronize()
     assert np.array_equal(d_lab.cpu().numpy().reshape(H, W), O.components(ref["graph"])), what + ": labels"
+
+
+@pytest.mark.parametrize("seed", list(range(60)))
+def test_random_view_matches_oracle(gpu, D, O, seed):
+    """dpx_stage_present on a random frame and view (cells or quads, scale
+    0.50..16.00, translation -40..40, window 1..256 x 1..192, antialiasing,
+    row order, with and without subdivision) equals orc_present."""
+    import ctypes
+    torch = gpu
+    rng = np.random.default_rng(7000 + seed)
+    kind = int(rng.choice([0, 1, 2, 4]))
+    ch = 4 if kind == 1 else 3
+    W, H = int(rng.integers(1, 49)), int(rng.integers(1, 41))
+    flags = (D.TOPDOWN if rng.random() < 0.4 else 0) | (D.ALPHA if kind == 1 else 0)
+    sub = bool(rng.random() < 0.5)
+    mode, sc = int(rng.integers(0, 2)), int(rng.integers(50, 1601))
+    tx, ty = int(rng.integers(-40, 41)), int(rng.integers(-40, 41))
+    ww, wh = int(rng.integers(1, 257)), int(rng.integers(1, 193))
+    vf = (D.VIEW_AA if rng.random() < 0.5 else 0) | (D.VIEW_TOPDOWN if rng.random() < 0.3 else 0)
+    what = f"case {seed}: kind {kind} {W}x{H} sub {sub} mode {mode} x{sc / 100} t ({tx},{ty}) {ww}x{wh} view {vf}"
+    print(what, flush=True)
+    img = D.synth_frame(kind, 7000 + seed, W, H, ch)
+    ref = O.depixelize(img, W, H, scale=1, subdivide=sub, channels=ch, flags=flags, want_diagram=True)
+    _, _, ec, diag = D.frame(img, W, H, ch, scale=1, subdivide=sub, flags=flags, want_diagram=True)
+    d_img = torch.from_numpy(img).to("cuda:0")
+    d_diag = torch.from_numpy(np.ascontiguousarray(diag)).to("cuda:0")
+    d_ec = torch.from_numpy(np.ascontiguousarray(ec)).to("cuda:0")
+    d_out = torch.zeros(wh * ww * 4, dtype=torch.uint8, device="cuda:0")
+    view = D.View(mode, sc, tx, ty, ww, wh, vf, 0)
+    D.check(D.lib.dpx_stage_present(d_img.data_ptr(), W, H, img.shape[1], ch, flags, d_diag.data_ptr(),
+                                    d_ec.data_ptr(), ctypes.byref(view), d_out.data_ptr(), ww * 4,
+                                    torch.cuda.current_stream().cuda_stream))
+    torch.cuda.synchronize()
+    want = O.present(img, W, H, ref["diagram"], ref["edge_count"], mode=mode, scale_x100=sc, tx=tx, ty=ty,
+                     width=ww, height=wh, view_flags=vf, channels=ch, flags=flags)
+    got = d_out.cpu().numpy().reshape(wh, ww, 4)
+    bad = np.argwhere((got != want).any(-1))
+    assert len(bad) == 0, f"{what}: {len(bad)} pixels differ, first at {bad[:3].tolist()}"

@@ -1,5 +1,6 @@
 """Randomized parity against the oracle: batches of random items, random
-frames through the per-frame stages, and random viewer windows.
+frames through the per-frame stages, random viewer windows, and random
+streams served to several threads at once.
 
 Each batch mixes 24 items of random kind (16-colour art, alpha sprites,
 curves, dense crossings), size (1..160 x 1..120), scale (1..15),
@@ -148,3 +149,53 @@ def test_random_view_matches_oracle(gpu, D, O, seed):
     got = d_out.cpu().numpy().reshape(wh, ww, 4)
     bad = np.argwhere((got != want).any(-1))
     assert len(bad) == 0, f"{what}: {len(bad)} pixels differ, first at {bad[:3].tolist()}"
+
+
+def test_random_streams_from_threads_match_oracle(gpu, D, O):
+    """Four threads, each with four streams of random size, scale, subdivision
+    and layout on one server, submit random frames in bursts of 1..8 and wait
+    for them; every result equals the oracle's (continuous batching mixes the
+    threads' requests on the device)."""
+    import threading
+    srv = D.Server(budget_bytes=8 << 30, max_batch=32)
+    errors, done = [], []
+    lock = threading.Lock()
+
+    def worker(wid):
+        rng = np.random.default_rng(300 + wid)
+        streams = []
+        for _ in range(4):
+            kind = int(rng.choice([0, 1, 2, 4]))
+            ch = 4 if kind == 1 else 3
+            W, H = int(rng.integers(1, 97)), int(rng.integers(1, 81))
+            S, sub = int(rng.integers(1, 9)), bool(rng.random() < 0.3)
+            flags = (D.TOPDOWN if rng.random() < 0.4 else 0) | (D.ALPHA if kind == 1 else 0)
+            streams.append((srv.open_stream(W, H, ch, S, sub, flags), kind, W, H, ch, S, sub, flags))
+        n = 0
+        for burst in range(25):
+            pend = []
+            for _ in range(int(rng.integers(1, 9))):
+                sid, kind, W, H, ch, S, sub, flags = streams[int(rng.integers(0, len(streams)))]
+                img = D.synth_frame(kind, int(rng.integers(0, 1 << 30)), W, H, ch)
+                out = np.zeros((H * S, W * S * 4), dtype=np.uint8)
+                pend.append((srv.submit(sid, img, out), img, out, (kind, W, H, ch, S, sub, flags)))
+            for t, img, out, (kind, W, H, ch, S, sub, flags) in pend:
+                rc = srv.wait(t)
+                ref = O.depixelize(img, W, H, scale=S, subdivide=sub, channels=ch, flags=flags)["rgba"]
+                if rc != 0 or not np.array_equal(out.reshape(H * S, W * S, 4), ref):
+                    with lock:
+                        errors.append((wid, burst, rc, kind, W, H, S, sub, flags))
+                n += 1
+        for st in streams:
+            srv.close_stream(st[0])
+        with lock:
+            done.append(n)
+
+    threads = [threading.Thread(target=worker, args=(w,)) for w in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    srv.close()
+    print("frames", sum(done), flush=True)
+    assert len(done) == 4 and not errors, f"{len(errors)} failed requests, first {errors[:3]}"

@@ -216,9 +216,10 @@ int dpx_context_reserve(dpx_context* ctx, int max_items, int64_t max_item_pixels
  * scratch (graph buffers, descriptors) is reused across calls: calls on one
  * context must be serialised by the caller (same stream or explicit sync).
  * A call whose items (and output alignment mod 16) equal the previous call's
- * only launches kernels on `stream`, so it may be captured into a HIP graph
- * (after one ordinary call with those items, and with timing off); the
- * graph reads d_in and writes d_out at replay time. */
+ * reuses its launch plan and only launches kernels on `stream`.  Do not
+ * capture dpx_batch_run into a HIP graph: a later call with other items
+ * rewrites the context's descriptors and may reallocate its scratch, which
+ * a captured graph would still point at.  Capture dpx_plan_run instead. */
 int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_out, const dpx_item* items,
                   int n_items, void* stream);
 /* Whether the batches that follow also write the resolved graph of every
@@ -240,6 +241,27 @@ typedef struct {
 } dpx_batch_times;
 int dpx_batch_timing(dpx_context* ctx, int enable);
 int dpx_batch_timing_read(dpx_context* ctx, dpx_batch_times* out);
+
+/* Plans: a batch layout validated once, with descriptors and graph scratch
+ * of its own (no counterpart in the reference, which renders one frame per
+ * call).  dpx_plan_run launches the batch's kernels for any input and output
+ * arena whose address modulo 16 equals out_align modulo 16; it allocates,
+ * copies and waits for nothing, so a caller may capture it into a HIP graph,
+ * and replays stay valid whatever else runs on the context, until the plan
+ * is destroyed.  With DPX_PLAN_GRAPH the library launches the kernels as a
+ * HIP graph of its own (built at the first run, its pointers updated when
+ * d_in or d_out change).  Runs of one plan must be serialised by the caller;
+ * plans must be destroyed before their context. */
+#define DPX_PLAN_GRAPH      1u  /* launch through a library-built HIP graph          */
+#define DPX_PLAN_KEEP_GRAPH 2u  /* also write the resolved graphs (dpx_plan_graph)   */
+typedef struct dpx_plan dpx_plan;
+int dpx_plan_create(dpx_context* ctx, const dpx_item* items, int n_items, uint64_t out_align, uint32_t flags,
+                    dpx_plan** out);
+int dpx_plan_run(dpx_plan* plan, const uint8_t* d_in, uint8_t* d_out, void* stream);
+/* The resolved graphs written by the plan's runs (device pointer + per-item
+ * offsets); DPX_EINVAL unless the plan was made with DPX_PLAN_KEEP_GRAPH. */
+int dpx_plan_graph(dpx_plan* plan, const uint8_t** d_graph, const uint64_t** offsets_host);
+void dpx_plan_destroy(dpx_plan* plan);
 
 /* ------------------------------------------------------------------------ */
 /* Serving: continuous batching of frames from many streams under an HBM     */

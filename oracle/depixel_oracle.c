@@ -96,15 +96,21 @@ uint32_t orc_rgb_to_yuv(uint32_t c)
 
 static int abs_i(int a) { return a < 0 ? -a : a; }
 
-/* The YUV comparison of diff() (graph_functions.cu:291-293): 1 = similar. */
-int orc_colors_similar(uint32_t ca, uint32_t cb)
+/* The comparison of diff() (graph_functions.cu:291-293) on two packed YUV
+ * words: the masked fields' differences against trY/trU/trV; 1 = similar. */
+int orc_yuv_similar(uint32_t y1, uint32_t y2)
 {
-    uint32_t y1 = orc_rgb_to_yuv(ca), y2 = orc_rgb_to_yuv(cb);
     int dy = (int)((y1 & YMASK) - (y2 & YMASK));
     int du = (int)((y1 & UMASK) - (y2 & UMASK));
     int dv = (int)((y1 & VMASK) - (y2 & VMASK));
     int different = (abs_i(dy) > TRY) || (abs_i(du) > TRU) || (abs_i(dv) > TRV);
     return !different;
+}
+
+/* The YUV comparison of diff() (graph_functions.cu:291-293): 1 = similar. */
+int orc_colors_similar(uint32_t ca, uint32_t cb)
+{
+    return orc_yuv_similar(orc_rgb_to_yuv(ca), orc_rgb_to_yuv(cb));
 }
 
 static const int EDX[8] = { -1, 0, 1, -1, 1, -1, 0, 1 };

@@ -43,6 +43,7 @@ typedef struct { float x, y; } orc_pt;
 /* ---- per-colour helpers (A2) ---- */
 uint32_t orc_rgb_to_yuv(uint32_t packed_c0c1c2);
 int      orc_colors_similar(uint32_t c_a, uint32_t c_b);
+int      orc_yuv_similar(uint32_t yuv_a, uint32_t yuv_b);
 
 /* ---- stages ---- */
 int orc_similarity_graph(const uint8_t* img, int W, int H, int stride, int channels,

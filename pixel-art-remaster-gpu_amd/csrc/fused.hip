@@ -554,7 +554,8 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             int x = x0 - 1 + k % RCX, y = y0 - 1 + k / RCX;
             if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
                 int row = topdown ? it.H - 1 - y : y;
-                uint32_t off = __umul24(row, it.stride) + __umul24(x, it.ch);  // operands < 2^24
+                // row * stride < 2^31 (checked on the host); the stride itself may exceed 2^24
+                uint32_t off = (uint32_t)row * (uint32_t)it.stride + __umul24(x, it.ch);
                 if (indexed) {
                     const uint32_t e = pal[base[off]];
                     s_col[k] = ((e >> (8 * o2)) & 0xFFu) | (e & 0xFF00u) | ((e >> (8 * o0)) & 0xFFu) << 16 |
@@ -1326,16 +1327,16 @@ extern "C" int dpx_batch_graph(dpx_context* ctx, const uint8_t** d_graph, const 
     return DPX_OK;
 }
 
-// Validates the items and builds the launch plan: device items, graph tiles,
-// raster groups by (scale, subdivide) and the descriptor blob, uploaded on
-// `stream` when it differs from the one already on the device.
-static int make_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* items, int n_items, hipStream_t stream)
+// Validates the items and builds a launch plan: device items, graph tiles,
+// raster groups by (scale, subdivision), the descriptor blob (offsets in the
+// plan refer to it) and the graph scratch bytes the batch needs.
+static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* items, int n_items, BatchPlan& plan,
+                      std::vector<uint8_t>& blob, std::vector<uint64_t>* g_offsets, size_t* graph_bytes)
 {
     using namespace dpx;
-    BatchPlan& plan = ctx->plan;
     plan.valid = false;
     std::vector<DevItem> dev(n_items);
-    ctx->g_offsets.assign(n_items, 0);
+    if (g_offsets) g_offsets->assign(n_items, 0);
     std::map<int, Group> groups;
     uint64_t goff = 0;
     for (int k = 0; k < n_items; k++) {
@@ -1373,7 +1374,7 @@ static int make_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* ite
         d.aligned16 = (a.out_stride % 16 == 0) && (a.out_offset % 16 == 0) &&
                       (reinterpret_cast<uintptr_t>(d_out) % 16 == 0);
         d.graph_w4 = a.width % 4 == 0;
-        ctx->g_offsets[k] = goff;
+        if (g_offsets) (*g_offsets)[k] = goff;
         goff += align_up((size_t)a.width * a.height, 256);
         int gkey = a.scale * 2 + d.subdivide;
         Group& grp = groups[gkey];
@@ -1384,9 +1385,6 @@ static int make_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* ite
     for (auto& kv : groups)
         if (int rc = ensure_mask(ctx, kv.second.scale)) return rc;
 
-    // graph arenas (grown only when a batch needs more)
-    if (int rc = grow_graph(ctx, goff)) return rc;
-
     // launch grids
     int g3d = 0;
     std::vector<uint2> gmap;
@@ -1394,21 +1392,43 @@ static int make_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* ite
     for (auto& kv : groups) kv.second.grid = tile_grid<RTX, RTY>(kv.second.dev, &kv.second.grid3d, &kv.second.map);
 
     // descriptor blob: items | graph tile map | per group (items, tile map)
-    std::vector<uint8_t> blob;
+    blob.clear();
     auto put = [&](const void* p, size_t n) {
         size_t off = align_up(blob.size(), 64);
         blob.resize(off + n);
         if (n) memcpy(blob.data() + off, p, n);
         return off;
     };
-    size_t off_items = put(dev.data(), dev.size() * sizeof(DevItem));
-    size_t off_gmap = put(gmap.data(), gmap.size() * sizeof(uint2));
-    std::vector<std::pair<size_t, size_t>> goffs;
+    plan.off_items = put(dev.data(), dev.size() * sizeof(DevItem));
+    plan.off_gmap = put(gmap.data(), gmap.size() * sizeof(uint2));
+    plan.ggrid = ggrid;
+    plan.g3d = g3d;
+    plan.groups.clear();
     for (auto& kv : groups) {
-        size_t a = put(kv.second.dev.data(), kv.second.dev.size() * sizeof(DevItem));
-        size_t b = put(kv.second.map.data(), kv.second.map.size() * sizeof(uint2));
-        goffs.push_back({ a, b });
+        const Group& grp = kv.second;
+        size_t a = put(grp.dev.data(), grp.dev.size() * sizeof(DevItem));
+        size_t b = put(grp.map.data(), grp.map.size() * sizeof(uint2));
+        plan.groups.push_back(PlanGroup{ grp.scale, grp.sub, grp.grid, grp.grid3d, a, b });
     }
+    plan.items.assign(items, items + n_items);
+    plan.out_mod = reinterpret_cast<uintptr_t>(d_out) % 16;
+    *graph_bytes = goff;
+    plan.valid = true;
+    return DPX_OK;
+}
+
+// The context's own plan for dpx_batch_run: built into the context's
+// descriptor buffer (uploaded on `stream` when it differs from the one
+// already on the device) and its graph scratch.
+static int make_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* items, int n_items, hipStream_t stream)
+{
+    BatchPlan& plan = ctx->plan;
+    std::vector<uint8_t> blob;
+    size_t gbytes = 0;
+    if (int rc = build_plan(ctx, d_out, items, n_items, plan, blob, &ctx->g_offsets, &gbytes)) return rc;
+    plan.valid = false;
+    // graph arenas (grown only when a batch needs more)
+    if (int rc = grow_graph(ctx, gbytes)) return rc;
     if (blob != ctx->last_desc) {
         if (ctx->upload_pending) {
             DPX_HIP(hipEventSynchronize(ctx->upload_done));
@@ -1421,20 +1441,34 @@ static int make_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* ite
         ctx->upload_pending = true;
         ctx->last_desc.swap(blob);
     }
-    plan.ggrid = ggrid;
-    plan.g3d = g3d;
-    plan.off_items = off_items;
-    plan.off_gmap = off_gmap;
-    plan.groups.clear();
-    size_t gi = 0;
-    for (auto& kv : groups) {
-        const Group& grp = kv.second;
-        plan.groups.push_back(PlanGroup{ grp.scale, grp.sub, grp.grid, grp.grid3d, goffs[gi].first, goffs[gi].second });
-        gi++;
-    }
-    plan.items.assign(items, items + n_items);
-    plan.out_mod = reinterpret_cast<uintptr_t>(d_out) % 16;
     plan.valid = true;
+    return DPX_OK;
+}
+
+// The kernels of one batch: kf_graph, then one kf_raster per (scale,
+// subdivision) group.  `desc` holds the plan's descriptor blob.
+struct LaunchArgs {
+    const uint8_t* desc;
+    uint8_t* gaux;
+    uint8_t* gfin;  // null: the resolved graph is not written
+};
+
+static int launch_plan(dpx_context* ctx, const BatchPlan& plan, const LaunchArgs& la, const uint8_t* d_in,
+                       uint8_t* d_out, hipStream_t stream, hipEvent_t ev_mid)
+{
+    const DevItem* d_items = reinterpret_cast<const DevItem*>(la.desc + plan.off_items);
+    const uint2* d_gmap = reinterpret_cast<const uint2*>(la.desc + plan.off_gmap);
+    hipLaunchKernelGGL(kf_graph, plan.ggrid, dim3(256), 0, stream, d_items, d_gmap, plan.g3d, d_in, la.gaux);
+    if (ev_mid) DPX_HIP(hipEventRecord(ev_mid, stream));
+    for (const PlanGroup& pg : plan.groups) {
+        RasterFn fn = pick_raster(pg.scale, pg.sub);
+        DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", pg.scale);
+        const DevItem* d_gitems = reinterpret_cast<const DevItem*>(la.desc + pg.off_items);
+        const uint2* d_rmap = reinterpret_cast<const uint2*>(la.desc + pg.off_map);
+        hipLaunchKernelGGL(fn, pg.grid, dim3(256), 0, stream, d_gitems, d_rmap, pg.grid3d, d_in, la.gaux, la.gfin,
+                           ctx->d_hull, ctx->d_mask[pg.scale], ctx->d_info[pg.scale], d_out);
+    }
+    DPX_HIP(hipGetLastError());
     return DPX_OK;
 }
 
@@ -1450,8 +1484,6 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
           plan.out_mod == reinterpret_cast<uintptr_t>(d_out) % 16 &&
           memcmp(plan.items.data(), items, (size_t)n_items * sizeof(dpx_item)) == 0))
         if (int rc = make_plan(ctx, d_out, items, n_items, stream)) return rc;
-    const DevItem* d_items = reinterpret_cast<const DevItem*>(ctx->d_desc + plan.off_items);
-    const uint2* d_gmap = reinterpret_cast<const uint2*>(ctx->d_desc + plan.off_gmap);
 
     hipEvent_t ev[3] = {};
     if (ctx->timing) {
@@ -1464,21 +1496,208 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         ctx->ev_used += 3;
         DPX_HIP(hipEventRecord(ev[0], stream));
     }
-    hipLaunchKernelGGL(kf_graph, plan.ggrid, dim3(256), 0, stream, d_items, d_gmap, plan.g3d, d_in, ctx->d_gaux);
-    if (ctx->timing) DPX_HIP(hipEventRecord(ev[1], stream));
-    uint8_t* gfin = ctx->keep_graph ? ctx->d_gfin : nullptr;
-    for (const PlanGroup& pg : plan.groups) {
-        RasterFn fn = pick_raster(pg.scale, pg.sub);
-        DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", pg.scale);
-        const DevItem* d_gitems = reinterpret_cast<const DevItem*>(ctx->d_desc + pg.off_items);
-        const uint2* d_rmap = reinterpret_cast<const uint2*>(ctx->d_desc + pg.off_map);
-        hipLaunchKernelGGL(fn, pg.grid, dim3(256), 0, stream, d_gitems, d_rmap, pg.grid3d, d_in, ctx->d_gaux, gfin,
-                           ctx->d_hull, ctx->d_mask[pg.scale], ctx->d_info[pg.scale], d_out);
-    }
+    LaunchArgs la{ ctx->d_desc, ctx->d_gaux, ctx->keep_graph ? ctx->d_gfin : nullptr };
+    if (int rc = launch_plan(ctx, plan, la, d_in, d_out, stream, ev[1])) return rc;
     ctx->last_kept = ctx->keep_graph;
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[2], stream));
-    DPX_HIP(hipGetLastError());
     return DPX_OK;
+}
+
+// ---------------------------------------------------------------------------
+// plans: a batch layout fixed once, with descriptors and graph scratch of its
+// own, launched directly or as a HIP graph the library builds
+// ---------------------------------------------------------------------------
+struct dpx_plan {
+    dpx_context* ctx = nullptr;
+    BatchPlan plan;
+    uint8_t* d_desc = nullptr;
+    uint8_t* d_gaux = nullptr;
+    uint8_t* d_gfin = nullptr;
+    uint32_t flags = 0;
+    std::mutex mu;
+    // DPX_PLAN_GRAPH: the kernels as graph nodes; the input and output
+    // pointers of the instantiated graph are updated in place when they change
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<hipGraphNode_t> nodes;
+    const uint8_t* g_in = nullptr;
+    uint8_t* g_out = nullptr;
+    std::vector<uint64_t> g_offsets;  // per item, in d_gfin
+};
+
+namespace {
+
+void plan_free(dpx_plan* p)
+{
+    if (!p) return;
+    if (p->exec) (void)hipGraphExecDestroy(p->exec);
+    if (p->graph) (void)hipGraphDestroy(p->graph);
+    (void)hipFree(p->d_desc);
+    (void)hipFree(p->d_gaux);
+    (void)hipFree(p->d_gfin);
+    delete p;
+}
+
+// Kernel node parameters of the plan's kernels for (d_in, d_out).  `store`
+// keeps the argument values the parameter pointers refer to.
+struct NodeArgs {
+    const DevItem* items;
+    const uint2* map;
+    int g3d;
+    const uint8_t* in;
+    const uint8_t* gaux_c;
+    uint8_t* gaux;
+    uint8_t* gfin;
+    const int8_t* hull;
+    const uint32_t* mask;
+    const uint16_t* info;
+    uint8_t* out;
+    void* ptrs[10];
+};
+
+hipKernelNodeParams node_params(dpx_plan* p, int k, const uint8_t* d_in, uint8_t* d_out, NodeArgs& a)
+{
+    hipKernelNodeParams np;
+    memset(&np, 0, sizeof(np));
+    np.blockDim = dim3(256);
+    a.in = d_in;
+    a.out = d_out;
+    a.gaux = p->d_gaux;
+    a.gaux_c = p->d_gaux;
+    if (k == 0) {
+        a.items = reinterpret_cast<const DevItem*>(p->d_desc + p->plan.off_items);
+        a.map = reinterpret_cast<const uint2*>(p->d_desc + p->plan.off_gmap);
+        a.g3d = p->plan.g3d;
+        void* v[5] = { &a.items, &a.map, &a.g3d, &a.in, &a.gaux };
+        memcpy(a.ptrs, v, sizeof(v));
+        np.func = reinterpret_cast<void*>(kf_graph);
+        np.gridDim = p->plan.ggrid;
+    } else {
+        const PlanGroup& pg = p->plan.groups[k - 1];
+        a.items = reinterpret_cast<const DevItem*>(p->d_desc + pg.off_items);
+        a.map = reinterpret_cast<const uint2*>(p->d_desc + pg.off_map);
+        a.g3d = pg.grid3d;
+        a.gfin = p->d_gfin;
+        a.hull = p->ctx->d_hull;
+        a.mask = p->ctx->d_mask[pg.scale];
+        a.info = p->ctx->d_info[pg.scale];
+        void* v[10] = { &a.items, &a.map, &a.g3d, &a.in, &a.gaux_c, &a.gfin, &a.hull, &a.mask, &a.info, &a.out };
+        memcpy(a.ptrs, v, sizeof(v));
+        np.func = reinterpret_cast<void*>(pick_raster(pg.scale, pg.sub));
+        np.gridDim = pg.grid;
+    }
+    np.kernelParams = a.ptrs;
+    return np;
+}
+
+int plan_build_graph(dpx_plan* p, const uint8_t* d_in, uint8_t* d_out)
+{
+    DPX_HIP(hipGraphCreate(&p->graph, 0));
+    const int nk = 1 + (int)p->plan.groups.size();
+    p->nodes.assign(nk, nullptr);
+    for (int k = 0; k < nk; k++) {
+        NodeArgs a;
+        memset(&a, 0, sizeof(a));
+        hipKernelNodeParams np = node_params(p, k, d_in, d_out, a);
+        DPX_CHECK(np.func, DPX_EINVAL, "no raster kernel for a group of the plan");
+        // every raster node depends on the graph node only
+        hipGraphNode_t dep = p->nodes[0];
+        DPX_HIP(hipGraphAddKernelNode(&p->nodes[k], p->graph, k ? &dep : nullptr, k ? 1 : 0, &np));
+    }
+    DPX_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
+    p->g_in = d_in;
+    p->g_out = d_out;
+    return DPX_OK;
+}
+
+}  // namespace
+
+extern "C" int dpx_plan_create(dpx_context* ctx, const dpx_item* items, int n_items, uint64_t out_align,
+                               uint32_t flags, dpx_plan** out)
+{
+    DPX_CHECK(out, DPX_EINVAL, "null output pointer");
+    *out = nullptr;
+    DPX_CHECK(ctx && items && n_items > 0, DPX_EINVAL, "bad arguments");
+    DPX_CHECK((flags & ~(uint32_t)(DPX_PLAN_GRAPH | DPX_PLAN_KEEP_GRAPH)) == 0, DPX_EINVAL, "unknown plan flags 0x%x",
+              flags);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    DPX_HIP(hipSetDevice(ctx->device));
+    auto* p = new dpx_plan();
+    p->ctx = ctx;
+    p->flags = flags;
+    std::vector<uint8_t> blob;
+    size_t gbytes = 0;
+    // out_align stands for d_out modulo 16 (only the alignment of the output
+    // arena enters the plan)
+    const uint8_t* fake_out = reinterpret_cast<const uint8_t*>((uintptr_t)(out_align % 16) + 16);
+    int rc = build_plan(ctx, fake_out, items, n_items, p->plan, blob, nullptr, &gbytes);
+    if (rc == DPX_OK && (hipMalloc(&p->d_desc, blob.size()) != hipSuccess ||
+                         hipMalloc(&p->d_gaux, std::max<size_t>(gbytes, 256)) != hipSuccess ||
+                         ((flags & DPX_PLAN_KEEP_GRAPH) &&
+                          hipMalloc(&p->d_gfin, std::max<size_t>(gbytes, 256)) != hipSuccess)))
+        rc = fail(DPX_ENOMEM, "plan allocation failed (%zu descriptor bytes, %zu graph bytes)", blob.size(), gbytes);
+    if (rc == DPX_OK && hipMemcpy(p->d_desc, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess)
+        rc = fail(DPX_EHIP, "descriptor upload failed");
+    if (rc != DPX_OK) {
+        plan_free(p);
+        return rc;
+    }
+    *out = p;
+    return DPX_OK;
+}
+
+extern "C" int dpx_plan_run(dpx_plan* p, const uint8_t* d_in, uint8_t* d_out, void* stream_)
+{
+    DPX_CHECK(p && d_in && d_out, DPX_EINVAL, "bad arguments");
+    DPX_CHECK(reinterpret_cast<uintptr_t>(d_out) % 16 == p->plan.out_mod, DPX_EINVAL,
+              "d_out %% 16 = %u, the plan was made for %u", (unsigned)(reinterpret_cast<uintptr_t>(d_out) % 16),
+              (unsigned)p->plan.out_mod);
+    std::lock_guard<std::mutex> lock(p->mu);
+    hipStream_t stream = as_stream(stream_);
+    DPX_HIP(hipSetDevice(p->ctx->device));
+    if (!(p->flags & DPX_PLAN_GRAPH)) {
+        LaunchArgs la{ p->d_desc, p->d_gaux, p->d_gfin };
+        return launch_plan(p->ctx, p->plan, la, d_in, d_out, stream, nullptr);
+    }
+    if (!p->exec) {
+        if (int rc = plan_build_graph(p, d_in, d_out)) return rc;
+    } else if (p->g_in != d_in || p->g_out != d_out) {
+        for (int k = 0; k < (int)p->nodes.size(); k++) {
+            NodeArgs a;
+            memset(&a, 0, sizeof(a));
+            hipKernelNodeParams np = node_params(p, k, d_in, d_out, a);
+            DPX_HIP(hipGraphExecKernelNodeSetParams(p->exec, p->nodes[k], &np));
+        }
+        p->g_in = d_in;
+        p->g_out = d_out;
+    }
+    DPX_HIP(hipGraphLaunch(p->exec, stream));
+    return DPX_OK;
+}
+
+extern "C" int dpx_plan_graph(dpx_plan* p, const uint8_t** d_graph, const uint64_t** offsets_host)
+{
+    DPX_CHECK(p && d_graph && offsets_host, DPX_EINVAL, "bad arguments");
+    DPX_CHECK(p->d_gfin, DPX_EINVAL, "the plan was made without DPX_PLAN_KEEP_GRAPH");
+    std::lock_guard<std::mutex> lock(p->mu);
+    if (p->g_offsets.empty()) {
+        uint64_t off = 0;
+        for (const dpx_item& a : p->plan.items) {
+            p->g_offsets.push_back(off);
+            off += align_up((size_t)a.width * a.height, 256);
+        }
+    }
+    *d_graph = p->d_gfin;
+    *offsets_host = p->g_offsets.data();
+    return DPX_OK;
+}
+
+extern "C" void dpx_plan_destroy(dpx_plan* p)
+{
+    if (!p) return;
+    (void)hipSetDevice(p->ctx->device);
+    (void)hipDeviceSynchronize();
+    plan_free(p);
 }
 
 extern "C" int dpx_batch_timing(dpx_context* ctx, int enable)

@@ -185,6 +185,17 @@ int grow_pinned(uint8_t** p, size_t* cap, size_t need)
 // pass through the lane's pinned staging buffers.
 int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
 {
+    // Once a copy or kernel is queued, every return waits for the lane's
+    // stream: the caller then releases the batch's arena ranges and host
+    // buffers, which work still in flight would otherwise write into.
+    struct DrainOnExit {
+        hipStream_t s;
+        bool armed = false;
+        ~DrainOnExit()
+        {
+            if (armed) (void)hipStreamSynchronize(s);
+        }
+    } drain{ lane.stream };
     std::vector<dpx_item> items;
     std::vector<Request*> live;
     std::vector<char> in_direct, out_direct;
@@ -211,6 +222,7 @@ int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
             src = lane.h_in + ho;
             ho += align256(r->in_bytes);
         }
+        drain.armed = true;
         DPX_HIP(hipMemcpyAsync(srv->d_arena + r->in_off, src, r->in_bytes, hipMemcpyHostToDevice, lane.stream));
         dpx_item it;
         memset(&it, 0, sizeof(it));
@@ -255,6 +267,7 @@ int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
         host_off.push_back(ho);
         ho += align256(r->out_bytes);
     }
+    drain.armed = false;
     DPX_HIP(hipStreamSynchronize(lane.stream));
     for (size_t k = 0; k < live.size(); k++) {
         if (out_direct[k]) continue;
@@ -407,6 +420,8 @@ extern "C" int dpx_server_open_stream(dpx_server* srv, int width, int height, in
               indexed ? "indexed streams have 1 channel" : "channels must be 3 or 4");
     DPX_CHECK(params->scale >= 1 && params->scale <= 15, DPX_EINVAL, "scale %d outside 1..15", params->scale);
     DPX_CHECK(params->reserved == 0, DPX_EINVAL, "reserved must be 0");
+    DPX_CHECK((int64_t)width * channels * height < (int64_t)1 << 31, DPX_EINVAL, "frame of %dx%dx%d exceeds 2 GiB",
+              width, height, channels);
     // size every lane for full batches of this stream's frames (outside the
     // server lock: growing a context waits for the device)
     const int64_t pixels = (int64_t)width * height;
@@ -469,6 +484,10 @@ extern "C" int dpx_server_submit(dpx_server* srv, int stream_id, const uint8_t* 
     const StreamInfo& si = it->second;
     DPX_CHECK(img && out, DPX_EINVAL, "null frame or output buffer");
     DPX_CHECK(stride >= si.W * si.ch, DPX_EINVAL, "stride %d < %d", stride, si.W * si.ch);
+    // the batched path's limit on an item's input (one bad frame must not
+    // fail the other requests of its batch)
+    DPX_CHECK((int64_t)stride * si.H < (int64_t)1 << 31, DPX_EINVAL, "stride %d x %d rows exceeds 2 GiB", stride,
+              si.H);
     DPX_CHECK(out_stride >= si.W * si.params.scale * 4, DPX_EINVAL, "out_stride %d too small", out_stride);
     auto r = std::make_unique<Request>();
     r->W = si.W;

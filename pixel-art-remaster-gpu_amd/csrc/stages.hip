@@ -121,7 +121,9 @@ __global__ void k_subdivide(ImgView im, const uint8_t* __restrict__ graph, dpx_p
     uint32_t node = graph[idx];
     if (node == 90) return;  // internal node (kernel.cu:231)
     int n = eaux[idx];
-    if (n < 3 || n > 2 * kMaxPts - 1) return;
+    // subdivide_cell emits up to 2n points into kMaxPoly slots; real hulls
+    // have at most 8 vertices, larger caller-supplied counts leave the cell
+    if (n < 3 || n > kMaxPoly / 2) return;
     int hx[2 * kMaxPts], hy[2 * kMaxPts];
     load_quarter(&daux[(size_t)idx * kCellSize], n, hx, hy);
     TjView tv = tj_view(im);

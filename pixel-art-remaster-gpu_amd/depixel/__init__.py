@@ -100,6 +100,10 @@ _SIGS = {
     "dpx_context_keep_graph": (c_int, [c_void_p, c_int]),
     "dpx_context_reserve": (c_int, [c_void_p, c_int, ctypes.c_int64, c_int]),
     "dpx_batch_timing": (c_int, [c_void_p, c_int]),
+    "dpx_plan_create": (c_int, [c_void_p, POINTER(Item), c_int, c_uint64, c_uint32, POINTER(c_void_p)]),
+    "dpx_plan_run": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p]),
+    "dpx_plan_graph": (c_int, [c_void_p, POINTER(c_void_p), POINTER(POINTER(c_uint64))]),
+    "dpx_plan_destroy": (None, [c_void_p]),
     "dpx_batch_timing_read": (c_int, [c_void_p, POINTER(BatchTimes)]),
     "dpx_server_create": (c_int, [c_int, c_uint64, c_int, POINTER(c_void_p)]),
     "dpx_server_destroy": (None, [c_void_p]),
@@ -252,6 +256,10 @@ class Context:
         arr = items if isinstance(items, ctypes.Array) else items_array(items)
         check(lib.dpx_batch_run(self.h, _ptr(d_in), _ptr(d_out), arr, len(arr), _stream(stream)))
 
+    def reserve(self, max_items: int, max_item_pixels: int, scale: int = 0):
+        """Sizes the context's buffers (and builds the tables of `scale`) ahead of the batches."""
+        check(lib.dpx_context_reserve(self.h, max_items, max_item_pixels, scale))
+
     def timing(self, enable: bool):
         check(lib.dpx_batch_timing(self.h, int(enable)))
 
@@ -270,6 +278,44 @@ class Context:
         offs = POINTER(c_uint64)()
         check(lib.dpx_batch_graph(self.h, ctypes.byref(p), ctypes.byref(offs)))
         return p.value, [offs[k] for k in range(n_items)]
+
+
+PLAN_GRAPH, PLAN_KEEP_GRAPH = 1, 2
+
+
+class Plan:
+    """A batch layout fixed once (dpx_plan): its own descriptors and graph
+    scratch, so runs can be captured into HIP graphs; with PLAN_GRAPH the
+    library launches it as a HIP graph itself."""
+
+    def __init__(self, ctx: Context, items, out_align: int = 0, flags: int = 0):
+        arr = items if isinstance(items, ctypes.Array) else items_array(items)
+        h = c_void_p()
+        check(lib.dpx_plan_create(ctx.h, arr, len(arr), out_align, flags, ctypes.byref(h)))
+        self.h = h
+        self.ctx = ctx  # plans go before their context
+        self.n = len(arr)
+
+    def run(self, d_in, d_out, stream=None):
+        check(lib.dpx_plan_run(self.h, _ptr(d_in), _ptr(d_out), _stream(stream)))
+
+    def graph(self):
+        """(device pointer of the resolved graphs, per-item byte offsets)."""
+        p = c_void_p()
+        offs = POINTER(c_uint64)()
+        check(lib.dpx_plan_graph(self.h, ctypes.byref(p), ctypes.byref(offs)))
+        return p.value, [offs[k] for k in range(self.n)]
+
+    def close(self):
+        if self.h:
+            lib.dpx_plan_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 class Server:

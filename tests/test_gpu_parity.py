@@ -131,6 +131,40 @@ def test_stage_entry_points_match_oracle(gpu, D, O):
     assert_equal(d_lab.cpu().numpy().reshape(H, W), O.components(g2), "component labels")
 
 
+def test_subdivide_stage_leaves_oversized_cells(gpu, D, O):
+    """dpx_stage_subdivide takes the edge counts from the caller: a cell with
+    17..31 points (more than subdivision's output slots hold at two points
+    per vertex; real hulls have at most 8) is left as it is."""
+    torch = gpu
+    W, H = 24, 20
+    img = D.synth_frame(0, 5, W, H, 3)
+    g = O.heuristic_cross(O.trivial_cross(O.similarity_graph(img, W, H)))
+    od, oec = O.cells(g)
+    dev = "cuda:0"
+    diag = od.reshape(W * H, CS, 2).copy()
+    ec = oec.reshape(-1).copy()
+    big = [k for k in range(W * H) if g.reshape(-1)[k] != 90][:15]
+    rng = np.random.default_rng(3)
+    for n, k in zip(range(17, 32), big):
+        ec[k] = n
+        diag[k, :n] = rng.integers(-1, 6, size=(n, 2)) * 0.25
+    d_img = torch.from_numpy(img.reshape(-1)).to(dev)
+    d_g = torch.from_numpy(g.reshape(-1).copy()).to(dev)
+    d_diag = torch.from_numpy(diag.reshape(-1).astype(np.float32)).to(dev)
+    d_daux = torch.zeros_like(d_diag)
+    d_ec = torch.from_numpy(ec.astype(np.int32)).to(dev)
+    d_eaux = torch.zeros_like(d_ec)
+    s = torch.cuda.current_stream()
+    D.check(D.lib.dpx_stage_subdivide(d_img.data_ptr(), W, H, img.shape[1], 3, 0, d_g.data_ptr(), d_diag.data_ptr(),
+                                      d_daux.data_ptr(), d_ec.data_ptr(), d_eaux.data_ptr(), s.cuda_stream))
+    torch.cuda.synchronize()
+    got_d = d_diag.cpu().numpy().reshape(W * H, CS, 2)
+    got_e = d_ec.cpu().numpy()
+    for n, k in zip(range(17, 32), big):
+        assert got_e[k] == n, f"cell {k}: count {got_e[k]} != {n}"
+        np.testing.assert_array_equal(got_d[k], diag[k], err_msg=f"cell {k} with {n} points changed")
+
+
 def test_launch_kernel_dropin(gpu, D, O):
     torch = gpu
     W, H = 64, 40

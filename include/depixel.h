@@ -284,8 +284,9 @@ int dpx_server_open_stream(dpx_server* srv, int width, int height, int channels,
                            const dpx_params* params, int* stream_id);
 int dpx_server_close_stream(dpx_server* srv, int stream_id);
 /* The palette of a DPX_INDEXED stream (channels 1): 256 four-byte entries,
- * copied by the server.  A frame is rendered with the palette set when it was
- * submitted (1 KiB that travels to the device with the frame).  Frames of an
+ * uploaded once into the server's HBM arena and kept resident there.  A frame
+ * is rendered with the palette set when it was submitted; a replaced palette
+ * is released when the last frame that names it completes.  Frames of an
  * indexed stream are refused until a palette is set. */
 int dpx_server_set_palette(dpx_server* srv, int stream_id, const uint8_t* palette);
 /* img: host frame (stride bytes/row); out: host RGBA buffer kept alive until
@@ -295,6 +296,9 @@ int dpx_server_set_palette(dpx_server* srv, int stream_id, const uint8_t* palett
 int dpx_server_submit(dpx_server* srv, int stream_id, const uint8_t* img, int stride,
                       uint8_t* out, int out_stride, uint64_t* ticket);
 int dpx_server_wait(dpx_server* srv, uint64_t ticket, int timeout_ms);
+/* dpx_server_wait that also returns the request's submit -> completion time
+ * (the latency the server's p50/p99 are taken over). */
+int dpx_server_wait_timed(dpx_server* srv, uint64_t ticket, int timeout_ms, double* latency_ms);
 int dpx_server_cancel(dpx_server* srv, uint64_t ticket);
 int dpx_server_stats_get(dpx_server* srv, dpx_server_stats* out);
 /* test hook: make the k-th submitted request (0-based) fail on the device side */
@@ -312,11 +316,20 @@ int dpx_host_free(void* p);
 /*           staircases, 4 checkerboard-dither regions, 30 isolated pixels   */
 /*   kind 1: sprite with alpha: 4-12 colours, dark 1-px outline, ~30%        */
 /*           transparent background, interior dither (channels must be 4)    */
-/*   kind 2: console frame: 16-64 colours, rectangles + 1-px random walks    */
+/*   kind 2: console frame: kind 0's content with 16-64 colours, plus 15% of */
+/*           the pixels in 1-px random walks of length 20-200              */
 /*   kind 4: large frame: 256 colours, 25% dither, diagonal lattice          */
 /* ------------------------------------------------------------------------ */
 int dpx_synth_frame(int kind, uint64_t seed, int width, int height, int channels, uint8_t* out,
                     int stride);
+/* Frame t (0..1000000) of synthetic game stream `seed`: palette indices
+ * (1 byte per pixel, row 0 first) of a background of 16x16 tiles and seven
+ * 16-24 px sprites that move by 1-4 px per axis each frame (about a tenth of
+ * the 16x16 tiles change between frames at 320x224); `palette` (1 KiB, or
+ * NULL) receives the stream's 16-32 colours as (c0, c1, c2, 255) entries,
+ * the same for every t.  Entries past the stream's colours are zero. */
+int dpx_synth_stream_frame(uint64_t seed, int t, int width, int height, uint8_t* out, int stride,
+                           uint8_t* palette);
 /* sprite size of sprite `index` of seed `seed` (sizes of the sprite mix) */
 int dpx_synth_sprite_size(uint64_t seed, int index, int* width, int* height);
 

@@ -14,6 +14,11 @@
 // its frames, so serving never waits for a context to grow.  Each frame's
 // result depends only on that frame (the kernels keep no state across
 // frames), so it is the same alone and in any mix.
+//
+// Palettes of indexed streams stay resident in the arena: setting a palette
+// uploads it once into a 1 KiB range, every frame submitted afterwards names
+// that range, and the range is released when the stream has moved on to
+// another palette (or closed) and no admitted or pending frame still names it.
 #include <string.h>
 
 #include <algorithm>
@@ -87,12 +92,19 @@ private:
     size_t used_ = 0;
 };
 
+// A palette resident in the arena, shared by the frames submitted while it
+// was the stream's current one.
+struct PaletteSlot {
+    size_t off = 0;  // arena offset of the 1 KiB
+    int users = 0;   // pending or admitted frames that name it
+    bool current = true;
+};
+
 struct StreamInfo {
     int W, H, ch;
     dpx_params params;
     bool open;
-    bool has_palette = false;                 // DPX_INDEXED streams
-    std::array<uint8_t, kPaletteBytes> palette{};
+    PaletteSlot* palette = nullptr;  // DPX_INDEXED streams: the current palette
 };
 
 struct Request {
@@ -106,12 +118,12 @@ struct Request {
     uint8_t* out = nullptr;
     int out_stride = 0;
     size_t in_bytes = 0, out_bytes = 0;
-    // DPX_INDEXED: the stream's palette when the frame was submitted; it
-    // travels after the frame, at in_off + pal_rel, in the input range
-    std::unique_ptr<std::array<uint8_t, kPaletteBytes>> palette;
-    size_t pal_rel = 0, in_alloc = 0;  // arena bytes of the input range (frame + palette)
-    size_t in_off = 0, out_off = 0;    // arena ranges while admitted
+    // DPX_INDEXED: the stream's palette when the frame was submitted
+    PaletteSlot* palette = nullptr;
+    size_t in_alloc = 0;             // arena bytes of the input range
+    size_t in_off = 0, out_off = 0;  // arena ranges while admitted
     Clock::time_point t_submit;
+    double latency_ms = 0;  // submit -> completion
     int status = 1;  // 1 pending, 2 admitted, else the final dpx_status
     bool inject = false;
 };
@@ -144,6 +156,7 @@ struct dpx_server {
     uint64_t peak = 0;
     uint64_t submitted = 0, completed = 0, failed = 0, rejected = 0, cancelled = 0, batches = 0, batched_frames = 0;
     std::vector<double> latencies_ms;
+    std::vector<std::unique_ptr<PaletteSlot>> palettes;
     Clock::time_point t_create;
     std::atomic<int64_t> reserved_pixels{ 0 };  // the largest stream frame the lanes are sized for
     Lane lanes[kLanes];
@@ -207,7 +220,6 @@ int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
         in_direct.push_back(in_pinned_block(r->img, r->in_bytes));
         out_direct.push_back(in_pinned_block(r->out, (rows - 1) * r->out_stride + row));
         if (!in_direct.back()) hin += align256(r->in_bytes);
-        if (r->palette) hin += align256(kPaletteBytes);
         if (!out_direct.back()) hout += align256(r->out_bytes);
     }
     if (live.empty()) return DPX_OK;
@@ -226,13 +238,7 @@ int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
         DPX_HIP(hipMemcpyAsync(srv->d_arena + r->in_off, src, r->in_bytes, hipMemcpyHostToDevice, lane.stream));
         dpx_item it;
         memset(&it, 0, sizeof(it));
-        if (r->palette) {
-            memcpy(lane.h_in + ho, r->palette->data(), kPaletteBytes);
-            DPX_HIP(hipMemcpyAsync(srv->d_arena + r->in_off + r->pal_rel, lane.h_in + ho, kPaletteBytes,
-                                   hipMemcpyHostToDevice, lane.stream));
-            ho += align256(kPaletteBytes);
-            it.palette_offset = r->in_off + r->pal_rel;
-        }
+        if (r->palette) it.palette_offset = r->palette->off;  // resident in the arena
         it.in_offset = r->in_off;
         it.out_offset = r->out_off;
         it.width = r->W;
@@ -277,6 +283,22 @@ int run_batch(dpx_server* srv, Lane& lane, const std::vector<Request*>& batch)
         for (size_t y = 0; y < rows; y++) memcpy(r->out + y * r->out_stride, src + y * row, row);
     }
     return DPX_OK;
+}
+
+// Drops a frame's hold on its palette slot; a slot that is no longer the
+// stream's current palette is released with its last frame (caller holds
+// the lock).
+void drop_palette(dpx_server* srv, PaletteSlot* p)
+{
+    if (!p) return;
+    p->users--;
+    if (p->users > 0 || p->current) return;
+    srv->arena.release(p->off, kPaletteBytes);
+    for (auto it = srv->palettes.begin(); it != srv->palettes.end(); ++it)
+        if (it->get() == p) {
+            srv->palettes.erase(it);
+            break;
+        }
 }
 
 // Admits the oldest pending requests that fit (caller holds the lock).
@@ -328,11 +350,13 @@ void lane_loop(dpx_server* srv, int li)
             for (Request* r : batch) {
                 srv->arena.release(r->in_off, r->in_alloc);
                 srv->arena.release(r->out_off, r->out_bytes);
+                drop_palette(srv, r->palette);
+                r->palette = nullptr;
                 r->status = r->inject ? DPX_EINJECTED : rc;
+                r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
                 if (r->status == DPX_OK) {
                     srv->completed++;
-                    srv->latencies_ms.push_back(
-                        std::chrono::duration<double, std::milli>(now - r->t_submit).count());
+                    srv->latencies_ms.push_back(r->latency_ms);
                 } else {
                     srv->failed++;
                 }
@@ -430,7 +454,7 @@ extern "C" int dpx_server_open_stream(dpx_server* srv, int width, int height, in
         DPX_CHECK(!srv->closing, DPX_ECLOSED, "server closing");
     }
     // a batch holds at most as many frames as the HBM budget fits
-    const uint64_t per_frame = align256((size_t)width * channels * height + (indexed ? kPaletteBytes + 3 : 0)) +
+    const uint64_t per_frame = align256((size_t)width * channels * height) +
                                align256((size_t)pixels * params->scale * params->scale * 4);
     const int items = (int)std::max<uint64_t>(1, std::min<uint64_t>(srv->max_batch, srv->budget / per_frame));
     for (Lane& lane : srv->lanes)
@@ -454,12 +478,33 @@ extern "C" int dpx_server_open_stream(dpx_server* srv, int width, int height, in
 extern "C" int dpx_server_set_palette(dpx_server* srv, int stream_id, const uint8_t* palette)
 {
     DPX_CHECK(srv && palette, DPX_EINVAL, "bad arguments");
-    std::lock_guard<std::mutex> lock(srv->mu);
+    std::unique_lock<std::mutex> lock(srv->mu);
     auto it = srv->streams.find(stream_id);
     DPX_CHECK(it != srv->streams.end() && it->second.open, DPX_ECLOSED, "stream %d is not open", stream_id);
     DPX_CHECK(it->second.params.flags & DPX_INDEXED, DPX_EINVAL, "stream %d is not indexed", stream_id);
-    memcpy(it->second.palette.data(), palette, kPaletteBytes);
-    it->second.has_palette = true;
+    // a 1 KiB range of the arena; while the arena is full, wait for batches
+    // to release theirs
+    size_t off = 0;
+    while (!srv->arena.alloc(kPaletteBytes, &off)) {
+        DPX_CHECK(!srv->closing, DPX_ECLOSED, "server closing");
+        srv->cv_done.wait(lock);
+        it = srv->streams.find(stream_id);
+        DPX_CHECK(it != srv->streams.end() && it->second.open, DPX_ECLOSED, "stream %d closed", stream_id);
+    }
+    srv->peak = std::max<uint64_t>(srv->peak, srv->arena.used());
+    if (hipMemcpy(srv->d_arena + off, palette, kPaletteBytes, hipMemcpyHostToDevice) != hipSuccess) {
+        srv->arena.release(off, kPaletteBytes);
+        return fail(DPX_EHIP, "palette upload failed");
+    }
+    srv->palettes.push_back(std::make_unique<PaletteSlot>());
+    PaletteSlot* slot = srv->palettes.back().get();
+    slot->off = off;
+    if (PaletteSlot* old = it->second.palette) {
+        old->current = false;
+        old->users++;  // (dropped at once unless frames still name it)
+        drop_palette(srv, old);
+    }
+    it->second.palette = slot;
     return DPX_OK;
 }
 
@@ -470,6 +515,12 @@ extern "C" int dpx_server_close_stream(dpx_server* srv, int stream_id)
     auto it = srv->streams.find(stream_id);
     DPX_CHECK(it != srv->streams.end() && it->second.open, DPX_EINVAL, "unknown stream %d", stream_id);
     it->second.open = false;  // requests already submitted still complete
+    if (PaletteSlot* old = it->second.palette) {
+        it->second.palette = nullptr;
+        old->current = false;
+        old->users++;
+        drop_palette(srv, old);
+    }
     return DPX_OK;
 }
 
@@ -496,12 +547,8 @@ extern "C" int dpx_server_submit(dpx_server* srv, int stream_id, const uint8_t* 
     r->params = si.params;
     r->in_bytes = (size_t)stride * si.H;
     r->in_alloc = r->in_bytes;
-    if (si.params.flags & DPX_INDEXED) {
-        DPX_CHECK(si.has_palette, DPX_EINVAL, "stream %d is indexed and has no palette yet", stream_id);
-        r->palette = std::make_unique<std::array<uint8_t, kPaletteBytes>>(si.palette);
-        r->pal_rel = (r->in_bytes + 3) / 4 * 4;
-        r->in_alloc = r->pal_rel + kPaletteBytes;
-    }
+    if (si.params.flags & DPX_INDEXED)
+        DPX_CHECK(si.palette, DPX_EINVAL, "stream %d is indexed and has no palette yet", stream_id);
     r->out_bytes = (size_t)si.W * si.params.scale * 4 * si.H * si.params.scale;
     if (align256(r->in_alloc) + align256(r->out_bytes) > srv->budget) {
         srv->rejected++;
@@ -517,6 +564,7 @@ extern "C" int dpx_server_submit(dpx_server* srv, int stream_id, const uint8_t* 
     r->out_stride = out_stride;
     r->t_submit = Clock::now();
     r->inject = std::find(srv->inject_at.begin(), srv->inject_at.end(), r->index) != srv->inject_at.end();
+    if ((r->palette = si.palette)) r->palette->users++;
     *ticket = r->ticket;
     srv->pending.push_back(r.get());
     srv->requests[r->ticket] = std::move(r);
@@ -526,6 +574,11 @@ extern "C" int dpx_server_submit(dpx_server* srv, int stream_id, const uint8_t* 
 }
 
 extern "C" int dpx_server_wait(dpx_server* srv, uint64_t ticket, int timeout_ms)
+{
+    return dpx_server_wait_timed(srv, ticket, timeout_ms, nullptr);
+}
+
+extern "C" int dpx_server_wait_timed(dpx_server* srv, uint64_t ticket, int timeout_ms, double* latency_ms)
 {
     DPX_CHECK(srv, DPX_EINVAL, "null server");
     std::unique_lock<std::mutex> lock(srv->mu);
@@ -537,6 +590,7 @@ extern "C" int dpx_server_wait(dpx_server* srv, uint64_t ticket, int timeout_ms)
     else if (!srv->cv_done.wait_for(lock, std::chrono::milliseconds(timeout_ms), done))
         return fail(DPX_ETIMEOUT, "ticket %llu still pending", (unsigned long long)ticket);
     int st = r->status;
+    if (latency_ms) *latency_ms = r->latency_ms;
     srv->requests.erase(it);
     if (st != DPX_OK) return fail(st, "request %llu failed (%d)", (unsigned long long)ticket, st);
     return DPX_OK;
@@ -552,6 +606,8 @@ extern "C" int dpx_server_cancel(dpx_server* srv, uint64_t ticket)
         Request* r = it->second.get();
         DPX_CHECK(r->status == 1, DPX_EINVAL, "ticket %llu is already admitted or done", (unsigned long long)ticket);
         srv->pending.erase(std::find(srv->pending.begin(), srv->pending.end(), r));
+        drop_palette(srv, r->palette);
+        r->palette = nullptr;
         r->status = DPX_ECANCELLED;
         srv->cancelled++;
     }

@@ -181,6 +181,99 @@ void sprite(Canvas& cv, SplitMix64& r)
         }
 }
 
+// ---------------------------------------------------------------------------
+// Game streams: palette-indexed frames of a scene whose background is a map
+// of 16x16 tiles (drawn from a tileset of the same content as kind 0 frames)
+// and whose sprites move by 1-4 pixels per axis from one frame to the next.
+// Frame t is a pure function of (seed, t): the sprites' paths are replayed
+// from frame 0.
+// ---------------------------------------------------------------------------
+constexpr int kStreamSprites = 7;
+
+struct StreamSprite {
+    int w, h, x, y, vx, vy;
+    std::vector<uint8_t> idx;  // w*h palette indices, 0 = transparent
+};
+
+int stream_palette(SplitMix64& r, uint8_t* pal1k)
+{
+    const int npal = r.uni(16, 32);
+    if (pal1k) {
+        memset(pal1k, 0, 1024);
+        for (int k = 0; k < npal; k++) {
+            for (int c = 0; c < 3; c++) pal1k[4 * k + c] = (uint8_t)r.uni(0, 255);
+            pal1k[4 * k + 3] = 255;
+        }
+    } else {
+        for (int k = 0; k < 3 * npal; k++) r.next();
+    }
+    return npal;
+}
+
+// Velocity component: +-1..4 pixels per frame.
+int stream_speed(SplitMix64& r) { return (r.uni(0, 1) ? 1 : -1) * r.uni(1, 4); }
+
+void stream_scene(uint64_t seed, int W, int H, std::vector<uint8_t>& bg, std::vector<StreamSprite>& sprites,
+                  uint8_t* pal1k)
+{
+    SplitMix64 r(seed * 0x2545F4914F6CDD1Dull + 0x3C6EF372FE94F82Bull);
+    const int npal = stream_palette(r, pal1k);
+    // identity palette in index form, so the frame drawers below write indices
+    std::vector<uint8_t> ids((size_t)npal);
+    for (int k = 0; k < npal; k++) ids[k] = (uint8_t)k;
+    // tileset: 8 tiles of 16x16 with rectangles, a staircase, dither, dots
+    constexpr int T = 16, NT = 8;
+    std::vector<uint8_t> tiles((size_t)NT * T * T);
+    for (int k = 0; k < NT; k++) {
+        Canvas tc{ T, T, 1, T, &tiles[(size_t)k * T * T] };
+        fill_rect(tc, 0, 0, T, T, &ids[r.uni(0, npal - 1)]);
+        rects(tc, r, ids, npal, r.uni(0, 3), 2, 12);
+        if (r.uni(0, 1)) lines(tc, r, ids, npal, 1);
+        if (r.uni(0, 3) == 0) dithers(tc, r, ids, npal, 1, 4, 12);
+        isolated(tc, r, ids, npal, r.uni(0, 3));
+    }
+    bg.assign((size_t)W * H, 0);
+    Canvas cv{ W, H, 1, W, bg.data() };
+    const int tx = (W + T - 1) / T, ty = (H + T - 1) / T;
+    for (int j = 0; j < ty; j++)
+        for (int i = 0; i < tx; i++) {
+            const uint8_t* tile = &tiles[(size_t)r.uni(0, NT - 1) * T * T];
+            for (int y = 0; y < T; y++)
+                for (int x = 0; x < T; x++) cv.set(i * T + x, j * T + y, &tile[y * T + x]);
+        }
+    // platforms across tiles
+    rects(cv, r, ids, npal, std::max(1, (int)(6.0 * W * H / (320.0 * 224.0))), 16, 96);
+    // sprites: an outlined body with detail, transparent around it
+    sprites.assign(kStreamSprites, StreamSprite{});
+    for (StreamSprite& s : sprites) {
+        s.w = r.uni(0, 1) ? 16 : 24;
+        s.h = r.uni(0, 2) ? 16 : 24;
+        s.w = std::min(s.w, W);
+        s.h = std::min(s.h, H);
+        s.x = r.uni(0, W - s.w);
+        s.y = r.uni(0, H - s.h);
+        s.vx = stream_speed(r);
+        s.vy = stream_speed(r);
+        s.idx.assign((size_t)s.w * s.h, 0);
+        const int body = r.uni(1, npal - 1), detail = r.uni(1, npal - 1), outline = r.uni(1, npal - 1);
+        const double cx = (s.w - 1) / 2.0, cy = (s.h - 1) / 2.0, ax = s.w / 2.0 * 0.95, ay = s.h / 2.0 * 0.95;
+        auto inside = [&](int x, int y) {
+            if (x < 0 || y < 0 || x >= s.w || y >= s.h) return false;
+            const double u = (x - cx) / ax, v = (y - cy) / ay;
+            return u * u + v * v <= 1.0;
+        };
+        const int dx0 = r.uni(0, s.w / 2), dy0 = r.uni(0, s.h / 2);
+        for (int y = 0; y < s.h; y++)
+            for (int x = 0; x < s.w; x++) {
+                if (!inside(x, y)) continue;
+                const bool edge = !inside(x - 1, y) || !inside(x + 1, y) || !inside(x, y - 1) || !inside(x, y + 1);
+                const bool det = x >= dx0 && x < dx0 + s.w / 2 && y >= dy0 && y < dy0 + s.h / 2 && ((x + y) & 1);
+                // index 0 stands for transparent: shift the body's indices by one
+                s.idx[(size_t)y * s.w + x] = (uint8_t)(1 + (edge ? outline : (det ? detail : body)));
+            }
+    }
+}
+
 const int kSpriteW[6] = { 8, 16, 16, 24, 32, 48 };
 const int kSpriteH[6] = { 8, 16, 32, 24, 32, 64 };
 
@@ -193,6 +286,39 @@ extern "C" int dpx_synth_sprite_size(uint64_t seed, int index, int* width, int* 
     int k = r.uni(0, 5);
     *width = kSpriteW[k];
     *height = kSpriteH[k];
+    return DPX_OK;
+}
+
+extern "C" int dpx_synth_stream_frame(uint64_t seed, int t, int width, int height, uint8_t* out, int stride,
+                                      uint8_t* palette)
+{
+    if (!out || width <= 0 || height <= 0 || stride < width || t < 0 || t > 1000000) return DPX_EINVAL;
+    std::vector<uint8_t> bg;
+    std::vector<StreamSprite> sprites;
+    stream_scene(seed, width, height, bg, sprites, palette);
+    // replay the sprites' paths: every frame each moves by its velocity
+    // (1-4 px per axis), bouncing off the borders, and picks a new velocity
+    // now and then
+    SplitMix64 r(seed * 0x9E3779B97F4A7C15ull + 0x7F4A7C159E3779B9ull);
+    for (int f = 1; f <= t; f++)
+        for (StreamSprite& s : sprites) {
+            if (r.uni(0, 23) == 0) {
+                s.vx = stream_speed(r);
+                s.vy = stream_speed(r);
+            }
+            int nx = s.x + s.vx, ny = s.y + s.vy;
+            if (nx < 0 || nx > width - s.w) { s.vx = -s.vx; nx = s.x + s.vx; }
+            if (ny < 0 || ny > height - s.h) { s.vy = -s.vy; ny = s.y + s.vy; }
+            s.x = std::clamp(nx, 0, width - s.w);
+            s.y = std::clamp(ny, 0, height - s.h);
+        }
+    for (int y = 0; y < height; y++) memcpy(out + (size_t)y * stride, &bg[(size_t)y * width], (size_t)width);
+    for (const StreamSprite& s : sprites)
+        for (int y = 0; y < s.h; y++)
+            for (int x = 0; x < s.w; x++) {
+                const uint8_t v = s.idx[(size_t)y * s.w + x];
+                if (v) out[(size_t)(s.y + y) * stride + s.x + x] = (uint8_t)(v - 1);
+            }
     return DPX_OK;
 }
 
@@ -220,10 +346,15 @@ extern "C" int dpx_synth_frame(int kind, uint64_t seed, int width, int height, i
             sprite(cv, r);
             return DPX_OK;
         case 2: {
+            // kind 0's content with 16-64 colours, and 15% of the pixels in
+            // long 1-px random walks
             int npal = r.uni(16, 64);
             auto pal = make_palette(r, npal, channels);
             fill_rect(cv, 0, 0, width, height, &pal[0]);
             rects(cv, r, pal, npal, std::max(1, (int)(40 * area)), 4, 64);
+            lines(cv, r, pal, npal, std::max(1, (int)(20 * area)));
+            dithers(cv, r, pal, npal, std::max(1, (int)(4 * area)), 8, 32);
+            isolated(cv, r, pal, npal, std::max(1, (int)(30 * area)));
             walks(cv, r, pal, npal, 0.15);
             return DPX_OK;
         }

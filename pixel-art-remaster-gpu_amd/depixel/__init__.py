@@ -112,6 +112,7 @@ _SIGS = {
     "dpx_server_set_palette": (c_int, [c_void_p, c_int, c_void_p]),
     "dpx_server_submit": (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, POINTER(c_uint64)]),
     "dpx_server_wait": (c_int, [c_void_p, c_uint64, c_int]),
+    "dpx_server_wait_timed": (c_int, [c_void_p, c_uint64, c_int, POINTER(c_double)]),
     "dpx_server_cancel": (c_int, [c_void_p, c_uint64]),
     "dpx_server_stats_get": (c_int, [c_void_p, POINTER(ServerStats)]),
     "dpx_server_inject_failure": (c_int, [c_void_p, c_uint64]),
@@ -120,6 +121,7 @@ _SIGS = {
     "dpx_host_free": (c_int, [c_void_p]),
     "dpx_synth_frame": (c_int, [c_int, c_uint64, c_int, c_int, c_int, c_void_p, c_int]),
     "dpx_synth_sprite_size": (c_int, [c_uint64, c_int, POINTER(c_int), POINTER(c_int)]),
+    "dpx_synth_stream_frame": (c_int, [c_uint64, c_int, c_int, c_int, c_void_p, c_int, c_void_p]),
 }
 for _name, (_res, _args) in _SIGS.items():
     if hasattr(lib, _name):
@@ -183,6 +185,19 @@ def pinned_empty(shape, dtype=np.uint8) -> np.ndarray:
     fin = weakref.finalize(buf, lib.dpx_host_free, p.value)
     fin.atexit = False  # at interpreter exit the process releases it
     return np.frombuffer(buf, dtype=dt).reshape(shape)
+
+
+def stream_frame(seed: int, t: int, width: int = 320, height: int = 224):
+    """Frame t of synthetic game stream `seed`: (indices (height, width) uint8, palette (256, 4) uint8)."""
+    idx = np.zeros((height, width), dtype=np.uint8)
+    pal = np.zeros((256, 4), dtype=np.uint8)
+    check(lib.dpx_synth_stream_frame(seed, t, width, height, _ptr(idx), width, _ptr(pal)))
+    return idx, pal
+
+
+def expand_indexed(idx: np.ndarray, pal: np.ndarray) -> np.ndarray:
+    """The 4-channel frame of an indexed frame's palette entries, (H, 4*W) uint8."""
+    return np.ascontiguousarray(pal[idx].reshape(idx.shape[0], idx.shape[1] * 4))
 
 
 def sprite_size(seed: int, index: int) -> tuple[int, int]:
@@ -369,6 +384,14 @@ class Server:
         if rc != ETIMEOUT:
             self._keep.pop(ticket, None)
         return rc
+
+    def wait_timed(self, ticket: int, timeout_ms: int = -1) -> tuple[int, float]:
+        """(status, submit -> completion ms) of a request."""
+        ms = c_double()
+        rc = lib.dpx_server_wait_timed(self.h, ticket, timeout_ms, ctypes.byref(ms))
+        if rc != ETIMEOUT:
+            self._keep.pop(ticket, None)
+        return rc, ms.value
 
     def cancel(self, ticket: int) -> int:
         return lib.dpx_server_cancel(self.h, ticket)

@@ -151,7 +151,9 @@ def test_indexed_server_stream_uses_the_palette_of_each_submit(gpu, D, O):
     ref_b = O.depixelize(exp_b, W, H, scale=s, channels=4)["rgba"]
     for k, o in enumerate(outs):
         assert_equal(o.reshape(H * s, W * s, 4), ref_a if k < 3 else ref_b, f"frame {k}")
-    # frames and results in pinned blocks move by DMA; the palette still travels with each frame
+    # frames and results in pinned blocks move by DMA; the palette stays resident
+    # on the device (only the current one: the replaced palette was released)
+    assert srv.stats().hbm_in_use == 1024
     pin_in = D.pinned_empty(idx.shape)
     pin_in[:] = idx
     pins = [D.pinned_empty((H * s, W * s * 4)) for _ in range(4)]

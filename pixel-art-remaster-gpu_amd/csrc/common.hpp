@@ -23,6 +23,12 @@ constexpr size_t kPaletteBytes = 1024;
 // Returns DPX_ENODEV when the process sees no GPU.
 int require_device();
 
+// Sizes a context's cell-row buffer (subdivided items) to at least `bytes`.
+int context_reserve_rows(dpx_context* ctx, size_t bytes);
+
+// Cell-row words per cell at scale S: 2S + 2 sample rows and a summary.
+constexpr size_t cell_row_words(int S) { return 2 * (size_t)S + 3; }
+
 }  // namespace dpx
 
 #define DPX_HIP(call)                                                                                   \

@@ -23,11 +23,10 @@
 // a 16-bit summary per key (kt_info, coverage_info); and at 1x and 2x into
 // each key's sample mask of each neighbour cell (kt_cell).
 //
-// This file is compiled twice.  Built with DPX_RASTER_SUB_TU it yields only
-// the raster kernels with subdivision (pick_raster_sub): their per-thread
-// polygon arrays must stay in memory rather than be promoted to registers,
-// which takes a compiler limit that would cost the other kernels.  Built
-// without it, it yields everything else.
+// With subdivision, kf_smooth runs between the two: it resolves the
+// crossings of each tile, smooths and ear-clips its non-internal cells
+// (polygons in LDS, each own cell once) and writes their coverage rows; the
+// raster (kf_raster<S, true>) then reads the resolved graph and those rows.
 #include <stdlib.h>
 #include <string.h>
 
@@ -62,12 +61,12 @@ struct DevItem {
     int32_t aligned16;  // output rows 16-byte aligned
     int32_t graph_w4;   // graph rows 4-byte aligned (W % 4 == 0)
     uint64_t pal_off;   // DPX_INDEXED: the palette's byte offset in the input arena
+    uint64_t rows_off;  // subdivision: byte offset of the item's cell rows (kf_smooth)
 };
 
 // ---------------------------------------------------------------------------
 // tables
 // ---------------------------------------------------------------------------
-#ifndef DPX_RASTER_SUB_TU
 __global__ void kt_hull(int8_t* __restrict__ table)
 {
     int key = blockIdx.x * blockDim.x + threadIdx.x;
@@ -81,7 +80,6 @@ __global__ void kt_hull(int8_t* __restrict__ table)
         e[9 + t] = (int8_t)(t < n ? hy[t] : 0);
     }
 }
-#endif
 
 // Coverage bits of one sample row of a polygon given in 1/64 units and
 // already ear-clipped (tri, nt).  Units of the test: 1/(128 S) of a cell.
@@ -104,92 +102,6 @@ __device__ inline uint32_t coverage_row(const int* x, const int* y, const uint8_
     return bits;
 }
 
-// ear_clip (geom.hpp) on a compacted copy of the polygon: each vertex packed
-// into one word (x in the low, y in the high 16 bits; 1/64 units fit) next
-// to its original index, removed vertices shifted out.  A vertex scan reads
-// one word per vertex instead of an index and two coordinates: the arrays
-// live in scratch memory here.  Same triangles, same failure rule.
-__device__ inline int ear_clip_compact(const int* x, const int* y, int n, uint8_t* tri)
-{
-    if (n < 3 || n > kMaxPoly) return 0;
-    uint32_t P[kMaxPoly];
-    uint8_t I[kMaxPoly];
-    int a2 = 0;
-    for (int p = n - 1, q = 0; q < n; p = q++) a2 += x[p] * y[q] - x[q] * y[p];
-    for (int v = 0; v < n; v++) {
-        const int k = a2 > 0 ? v : n - 1 - v;
-        P[v] = ((uint32_t)x[k] & 0xFFFFu) | ((uint32_t)y[k] << 16);
-        I[v] = (uint8_t)k;
-    }
-    auto px = [](uint32_t p) { return (int)(int16_t)(p & 0xFFFFu); };
-    auto py = [](uint32_t p) { return (int)p >> 16; };
-    int nv = n, ntri = 0, count = 2 * nv;
-    for (int v = nv - 1; nv > 2;) {
-        if (0 >= (count--)) return ntri;
-        int u = v;
-        if (nv <= u) u = 0;
-        v = u + 1;
-        if (nv <= v) v = 0;
-        int w = v + 1;
-        if (nv <= w) w = 0;
-        const uint32_t A = P[u], B = P[v], Cc = P[w];
-        const int ax = px(A), ay = py(A), bx = px(B), by = py(B), cx = px(Cc), cy = py(Cc);
-        bool ear = ((bx - ax) * (cy - ay) - (by - ay) * (cx - ax)) > 0;
-        if (ear) {
-            for (int p = 0; p < nv; p++) {
-                if (p == u || p == v || p == w) continue;
-                const uint32_t Q = P[p];
-                if (inside_tri(ax, ay, bx, by, cx, cy, px(Q), py(Q))) { ear = false; break; }
-            }
-        }
-        if (ear) {
-            tri[3 * ntri + 0] = I[u];
-            tri[3 * ntri + 1] = I[v];
-            tri[3 * ntri + 2] = I[w];
-            ntri++;
-            for (int s = v, t = v + 1; t < nv; s++, t++) {
-                P[s] = P[t];
-                I[s] = I[t];
-            }
-            nv--;
-            count = 2 * nv;
-        }
-    }
-    return ntri;
-}
-
-// All mask_w(S) rows of coverage_row at once, with each triangle's vertices
-// read and oriented once (the polygon arrays live in memory, not registers,
-// so coverage_row's per-row re-reads dominate), each row's covered columns
-// from the edges' intervals (tri_row_bits) rather than a test per sample.
-// Same bits as coverage_row.
-template <int S>
-__device__ inline void coverage_rows(const int* x, const int* y, const uint8_t* tri, int nt, uint32_t* rows)
-{
-    constexpr int kmin = mask_kmin(S), mw = mask_w(S);
-#pragma unroll
-    for (int r = 0; r < mw; r++) rows[r] = 0;
-    for (int t = 0; t < nt; t++) {
-        int ax = x[tri[3 * t]] * 2 * S, ay = y[tri[3 * t]] * 2 * S;
-        int bx = x[tri[3 * t + 1]] * 2 * S, by = y[tri[3 * t + 1]] * 2 * S;
-        int cx = x[tri[3 * t + 2]] * 2 * S, cy = y[tri[3 * t + 2]] * 2 * S;
-        const int o = (bx - ax) * (cy - ay) - (by - ay) * (cx - ax);
-        if (o == 0) continue;  // degenerate: covers nothing (tri_covers)
-        if (o < 0) {
-            int tx = bx, ty = by;
-            bx = cx; by = cy; cx = tx; cy = ty;
-        }
-        const int ymin = min(ay, min(by, cy)), ymax = max(ay, max(by, cy));
-#pragma unroll
-        for (int r = 0; r < mw; r++) {
-            const int sy = (2 * (kmin + r) + 1) * 64;
-            if (sy < ymin || sy > ymax) continue;
-            rows[r] |= tri_row_bits(ax, ay, bx, by, cx, cy, sy, kmin, mw);  // the edge_in bits of the row
-        }
-    }
-}
-
-#ifndef DPX_RASTER_SUB_TU
 __global__ void kt_mask(const int8_t* __restrict__ hull_table, int S, uint32_t* __restrict__ mask)
 {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -205,7 +117,6 @@ __global__ void kt_mask(const int8_t* __restrict__ hull_table, int S, uint32_t* 
     int nt = ear_clip(x, y, n, tri, &ok);
     mask[gid] = coverage_row(x, y, tri, nt, S, r);
 }
-#endif
 
 // Summary of a cell's coverage rows at scale S: bit 0 = it covers every
 // sample of its own square; bit 1 + (dy+1)*3 + (dx+1) = it covers a sample of
@@ -235,7 +146,6 @@ __host__ __device__ inline uint32_t coverage_info(const uint32_t* rows, int S)
 // it is the candidate at (dx, dy) of that cell.  They follow the row table
 // in the same allocation.
 constexpr int kMaxCellScale = 2;
-#ifndef DPX_RASTER_SUB_TU
 __global__ void kt_cell(const uint32_t* __restrict__ mask, int S, uint16_t* __restrict__ cell)
 {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -253,16 +163,13 @@ __global__ void kt_cell(const uint32_t* __restrict__ mask, int S, uint16_t* __re
     }
     cell[gid] = (uint16_t)m;
 }
-#endif
 
-#ifndef DPX_RASTER_SUB_TU
 __global__ void kt_info(const uint32_t* __restrict__ mask, int S, uint16_t* __restrict__ info)
 {
     int key = blockIdx.x * blockDim.x + threadIdx.x;
     if (key >= kKeys) return;
     info[key] = (uint16_t)coverage_info(mask + (size_t)key * mask_w(S), S);
 }
-#endif
 
 // A cell whose own square it covers fully, and into which none of the four
 // cells drawn after it (up-left, up, up-right, right) reaches, shows its own
@@ -312,7 +219,6 @@ __device__ inline TileAt locate_tile(const uint2* __restrict__ tile_map, int gri
     }
 }
 
-#ifndef DPX_RASTER_SUB_TU
 // ---------------------------------------------------------------------------
 // kf_graph: similarity graph + trivial crossing removal
 // ---------------------------------------------------------------------------
@@ -332,6 +238,9 @@ __device__ inline TileAt locate_tile(const uint2* __restrict__ tile_map, int gri
 //      from those bits of the neighbouring bytes.
 constexpr int GTX = 64, GTY = 32;
 constexpr int GLX = GTX + 3, GLY = GTY + 4;  // lane words
+// Lane-word rows in LDS: GLW words, column c + 3 holding x = x0 - 1 + c (so
+// that x0 - 4 .. x0 + 67, 18 aligned groups of 4 pixels, fit a row).
+constexpr int GLW = GTX + 8;
 constexpr int GHX = GTX + 2, GHY = GTY + 2;  // positions of pass 2
 constexpr int GHP = GTX + 8;                 // byte pitch of pass 2's rows: column c is x = x0 - 4 + c
 
@@ -339,7 +248,7 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
                                                 int grid3d, const uint8_t* __restrict__ in,
                                                 uint8_t* __restrict__ gaux)
 {
-    __shared__ uint32_t s_lane[GLX * GLY];
+    __shared__ __align__(16) uint32_t s_lane[GLW * GLY];
     __shared__ uint32_t s_h32[GHY * GHP / 4];
     uint8_t* s_h = reinterpret_cast<uint8_t*>(s_h32);
     const int t = threadIdx.x;
@@ -370,10 +279,58 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         if (alpha && base[off + 3] == 0) return kLaneTransparent;
         return yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque;
     };
-    // Window column c (x = x0 - 1 + c), row r (y = y0 - 2 + r).  Columns
-    // 0..63: one per thread (t % 64), every fourth row from t / 64, the byte
-    // offset stepping by four rows; columns 64..GLX-1: one word per thread.
-    {
+    // Rows of 3 or 4 channels that start on 4-byte boundaries are read as
+    // groups of four pixels (3 or 4 dwords per group): x0 - 4 .. x0 + 67,
+    // GLY rows, 18 groups a row; groups that straddle the frame's left or
+    // right edge are read pixel by pixel.
+    const bool grouped = !indexed && ((it.in_off | (uint64_t)it.stride) & 3) == 0;
+    if (grouped) {
+        for (int k = t; k < GLY * 18; k += 256) {
+            const int r = k / 18, g = k % 18;
+            const int x = x0 - 4 + 4 * g, y = y0 - 2 + r;
+            uint32_t lw[4] = { 0, 0, 0, 0 };
+            if ((unsigned)y < (unsigned)it.H && x < it.W && x + 4 > 0) {
+                const uint32_t rowoff = (uint32_t)(topdown ? it.H - 1 - y : y) * (uint32_t)it.stride;
+                if (x >= 0 && x + 3 < it.W) {
+                    const uint32_t* p = reinterpret_cast<const uint32_t*>(base + rowoff + (uint32_t)x * it.ch);
+                    uint32_t d[4];
+                    d[0] = p[0];
+                    d[1] = p[1];
+                    d[2] = p[2];
+                    d[3] = it.ch == 4 ? p[3] : 0u;
+                    auto byte = [&](int j) { return (d[j >> 2] >> (8 * (j & 3))) & 0xFFu; };
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        int c0, c1, c2;
+                        bool clear = false;
+                        if (it.ch == 3) {  // (constant byte indices; the order is a select)
+                            const int bA = (int)byte(3 * q), bC = (int)byte(3 * q + 2);
+                            c0 = rgb ? bC : bA;
+                            c1 = (int)byte(3 * q + 1);
+                            c2 = rgb ? bA : bC;
+                        } else {
+                            c0 = (int)((d[q] >> (8 * o0)) & 0xFFu);
+                            c1 = (int)((d[q] >> 8) & 0xFFu);
+                            c2 = (int)((d[q] >> (8 * o2)) & 0xFFu);
+                            clear = alpha && (d[q] >> 24) == 0;
+                        }
+                        lw[q] = clear ? kLaneTransparent : (yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque);
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 4; q++)
+                        if ((unsigned)(x + q) < (unsigned)it.W) lw[q] = lane_word(rowoff + (uint32_t)(x + q) * it.ch);
+                }
+            }
+            u32x4 v = { lw[0], lw[1], lw[2], lw[3] };
+            *reinterpret_cast<u32x4*>(&s_lane[r * GLW + 4 * g]) = v;
+        }
+    }
+    // Otherwise pixel by pixel: window column c (x = x0 - 1 + c), row r
+    // (y = y0 - 2 + r).  Columns 0..63: one per thread (t % 64), every fourth
+    // row from t / 64, the byte offset stepping by four rows; columns
+    // 64..GLX-1: one word per thread.
+    if (!grouped) {
         const int c = t & 63, x = x0 - 1 + c;
         const bool xin = (unsigned)x < (unsigned)it.W;
         int y = y0 - 2 + (t >> 6);
@@ -383,22 +340,22 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         for (int r = t >> 6; r < GLY; r += 4, y += 4, off += step) {
             uint32_t lw = 0;
             if (xin && (unsigned)y < (unsigned)it.H) lw = lane_word(off);
-            s_lane[r * GLX + c] = lw;
+            s_lane[r * GLW + c + 3] = lw;
         }
     }
-    if (t < (GLX - 64) * GLY) {
+    if (!grouped && t < (GLX - 64) * GLY) {
         const int c = 64 + t % (GLX - 64), r = t / (GLX - 64);
         const int x = x0 - 1 + c, y = y0 - 2 + r;
         uint32_t lw = 0;
         if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
             lw = lane_word((uint32_t)((topdown ? it.H - 1 - y : y) * it.stride + x * it.ch));
-        s_lane[r * GLX + c] = lw;
+        s_lane[r * GLW + c + 3] = lw;
     }
     __syncthreads();
     // pass 2: position (x0 - 1 + cx, y0 - 1 + cy), same column split
     auto edges = [&](int cx, int cy) {
-        const int c = (cy + 1) * GLX + cx;  // its lane word
-        uint32_t up = s_lane[c + GLX], ur = s_lane[c + GLX + 1], rt = s_lane[c + 1], dr = s_lane[c - GLX + 1];
+        const int c = (cy + 1) * GLW + cx + 3;  // its lane word
+        uint32_t up = s_lane[c + GLW], ur = s_lane[c + GLW + 1], rt = s_lane[c + 1], dr = s_lane[c - GLW + 1];
         LaneRef a = lane_ref(s_lane[c]);
         uint32_t h = lane_similar(a, up) << 1 | lane_similar(a, ur) << 2 | lane_similar(a, rt) << 4 |
                      lane_similar(a, dr) << 7;
@@ -444,14 +401,12 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         }
     }
 }
-#endif  // DPX_RASTER_SUB_TU
 
 // ---------------------------------------------------------------------------
 // kf_raster: heuristic crossing resolution, cells (+ subdivision) and the
 // gather raster of a 32x8-cell tile
 // ---------------------------------------------------------------------------
 constexpr int RTX = 32, RTY = 8;
-constexpr int RHX = RTX + 4, RHY = RTY + 4;  // hulls, halo 2 (subdivision)
 constexpr int RCX = RTX + 2, RCY = RTY + 2;  // candidate cells, halo 1
 constexpr uint16_t kNoCell = 0xFFFF;
 
@@ -464,68 +419,22 @@ constexpr uint16_t kNoCell = 0xFFFF;
 // snapshot rows above and below it.
 constexpr int RWW = 12, RWB = 4 * RWW;
 static_assert(RTX * RTY == 256, "one own cell per thread of a 256-thread block");
-template <bool SUB>
-struct RasterWin {
-    static constexpr int GHY = SUB ? 2 : 1;               // resolved rows y0 - GHY .. y0 + RTY - 1 + GHY
-    static constexpr int GY = RTY + 2 * GHY, AY = GY + 2;  // snapshot rows: one more on each side
-};
+// Resolved rows y0 - 1 .. y0 + RTY (candidate cells need their left and
+// right neighbours' bytes for their keys); resolving them reads one more
+// snapshot row on each side.
+constexpr int GY = RTY + 2, AY = GY + 2;
 
-// `items` are the items of one (scale, subdivision) group, in launch order.
-template <int S, bool SUB>
-__global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
-                                                 int grid3d,
-                                                 const uint8_t* __restrict__ in, const uint8_t* __restrict__ gaux,
-                                                 uint8_t* __restrict__ gfin, const int8_t* __restrict__ hull_table,
-                                                 const uint32_t* __restrict__ mask_table,
-                                                 const uint16_t* __restrict__ info_table, uint8_t* __restrict__ out)
+// Per-cell coverage rows of smoothed cells (kf_smooth -> kf_raster<S, true>):
+// MW words of sample rows and one word of coverage summary per cell, cell
+// (x, y) of an item at word (y * W + x) * (MW + 1) of its range.
+__host__ __device__ constexpr int rows_words(int S) { return mask_w(S) + 1; }
+
+// The snapshot rows ay0 .. ay0 + AY - 1 of a tile's window into LDS.  With
+// W % 4 == 0 the graph rows are word aligned and a word is either wholly
+// inside the frame or wholly outside.
+__device__ inline void tile_snapshot_load(const DevItem& it, const uint8_t* __restrict__ aux, int x0, int ay0,
+                                          uint32_t* s_aux32, int t)
 {
-    using Win = RasterWin<SUB>;
-    constexpr int GY = Win::GY, AY = Win::AY, GHY = Win::GHY;
-    constexpr int MW = mask_w(S), KMIN = mask_kmin(S);
-    __shared__ uint32_t s_aux32[AY * RWW];
-    __shared__ uint32_t s_p32[(GY + 1) * RWW];   // decisions of crossed blocks, by origin (snapshot rows 0..GY)
-    __shared__ uint16_t s_blk[(GY + 1) * 40];   // crossed blocks: origin byte index in the window
-    __shared__ int s_nblk;
-    __shared__ uint32_t s_g32[GY * RWW];
-    __shared__ uint16_t s_key[RCX * RCY];
-    __shared__ uint8_t s_info[RCX * RCY];  // coverage summary bits 0..4 (simple_cell)
-    __shared__ uint32_t s_col[RCX * RCY];
-    __shared__ uint32_t s_mask[SUB ? RCX * RCY * MW : 1];  // coverage rows of the smoothed candidates
-    __shared__ int8_t s_hull[SUB ? RHX * RHY * 17 : 1];
-    __shared__ uint16_t s_list[RTX * RTY];  // own cells that need the candidate search
-    __shared__ int s_nlist;
-    // resolved pixels of the first listed cells, so that every own
-    // cell's rows are stored together (a line is then written by one store
-    // instruction instead of being split between the direct and searched
-    // cells).  A pixel is kept as its colour; from S = 6 a tile whose listed
-    // cells would overflow that keeps the code of each pixel's winning
-    // candidate instead (1 byte: (1 - dy) * 3 + (1 - dx), 9 = clear colour),
-    // four times as many cells.  Measured: codes cost a decode per pixel, so
-    // tiles that fit keep colours; at S = 5 codes lose to the overflow stores.
-    constexpr bool CODES = S >= 6;
-    constexpr int PX_WORDS = 2048, PX_CELLS = PX_WORDS / (S * S), PX_CODE_CELLS = 4 * PX_WORDS / (S * S);
-    static_assert(PX_CELLS >= 1, "at least one buffered cell");
-    __shared__ uint32_t s_px[PX_WORDS];
-    uint8_t* s_px8 = reinterpret_cast<uint8_t*>(s_px);
-    __shared__ uint16_t s_idx[RTX * RTY];    // list index of a listed own cell
-    __shared__ uint8_t s_simple[RTX * RTY];  // own cell shows its own colour only
-    const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
-
-    const int t = threadIdx.x;
-    const TileAt at = locate_tile<8>(tile_map, grid3d);
-    const DevItem it = items[at.item];
-    const int x0 = at.tx * RTX, y0 = at.ty * RTY;
-    const uint8_t* aux = gaux + it.g_off;
-    const int ay0 = y0 - GHY - 1;  // first snapshot row
-
-    if (t == 0) {
-        s_nlist = 0;
-        s_nblk = 0;
-    }
-    if (t < (GY + 1) * RWW) s_p32[t] = 0;
-    // The snapshot bytes and the candidates' colours are independent loads:
-    // issue them together.  With W % 4 == 0 the graph rows are word aligned
-    // and a word is either wholly inside the frame or wholly outside.
     if (it.graph_w4) {
         if (t < AY * RWW) {
             int r = t / RWW, w = t % RWW;
@@ -544,39 +453,23 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             s8[k] = ok ? aux[(uint32_t)(y * it.W + x)] : 0;
         }
     }
-    {
-        const uint8_t* base = in + it.in_off;
-        const uint32_t o0 = (it.flags & DPX_RGB) ? 2 : 0, o2 = 2 - o0;
-        const bool topdown = it.flags & DPX_TOPDOWN, four = it.ch == 4;
-        const bool indexed = it.flags & DPX_INDEXED;
-        const uint32_t* pal = reinterpret_cast<const uint32_t*>(in + it.pal_off);
-        for (int k = t; k < RCX * RCY; k += 256) {
-            int x = x0 - 1 + k % RCX, y = y0 - 1 + k / RCX;
-            if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
-                int row = topdown ? it.H - 1 - y : y;
-                // row * stride < 2^31 (checked on the host); the stride itself may exceed 2^24
-                uint32_t off = (uint32_t)row * (uint32_t)it.stride + __umul24(x, it.ch);
-                if (indexed) {
-                    const uint32_t e = pal[base[off]];
-                    s_col[k] = ((e >> (8 * o2)) & 0xFFu) | (e & 0xFF00u) | ((e >> (8 * o0)) & 0xFFu) << 16 |
-                               (e & 0xFF000000u);
-                } else {
-                    uint32_t a = four ? base[off + 3] : 255u;
-                    s_col[k] = (uint32_t)base[off + o2] | (uint32_t)base[off + 1] << 8 |
-                               (uint32_t)base[off + o0] << 16 | a << 24;  // colorPos (kernel.cu:98-101)
-                }
-            }
-        }
-    }
-    __syncthreads();
-    // Heuristic crossing resolution (ambiguous_cross_Kernel).  Only the
-    // pixels of crossed 2x2 blocks (both diagonals present) change, and in
-    // the symmetric snapshot the block with lower-left pixel p is crossed iff
-    // p has bit 2 and its right neighbour bit 0.  1: list the crossed blocks
-    // of the window, four origins of a row per thread; 2: decide each listed
-    // block once (processHeuristics2), the walks reading the snapshot in LDS
-    // where they can and in HBM beyond the window; 3: remove the losing
-    // diagonals, four positions per thread.
+}
+
+// Heuristic crossing resolution (ambiguous_cross_Kernel) of a tile's window,
+// after tile_snapshot_load and a barrier.  Only the pixels of crossed 2x2
+// blocks (both diagonals present) change, and in the symmetric snapshot the
+// block with lower-left pixel p is crossed iff p has bit 2 and its right
+// neighbour bit 0.  1: list the crossed blocks of the window, four origins
+// of a row per thread; 2: decide each listed block once
+// (processHeuristics2), the walks reading the snapshot in LDS where they can
+// and in HBM beyond the window; 3: remove the losing diagonals, four
+// positions per thread, into s_g32 (resolved rows y0 - 1 .. y0 + RTY), and
+// the tile's own resolved bytes into gfin when it is given.  s_p32 must
+// have been cleared and *s_nblk zeroed before the barrier.
+__device__ inline void tile_resolve(const DevItem& it, const uint8_t* __restrict__ aux, uint8_t* __restrict__ gfin,
+                                    int x0, int y0, int ay0, const uint32_t* s_aux32, uint32_t* s_p32,
+                                    uint16_t* s_blk, int* s_nblk, uint32_t* s_g32, int t)
+{
     for (int k = t; k < (GY + 1) * 10; k += 256) {
         const int r = k / 10, w = k % 10 + 1;  // word w of snapshot row r: origins of blocks on rows r, r + 1
         const uint32_t* a = s_aux32 + r * RWW + w;
@@ -585,7 +478,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
         while (cross) {
             int q = __builtin_ctz(cross) >> 3;
             cross &= cross - 1;
-            s_blk[atomicAdd(&s_nblk, 1)] = (uint16_t)(r * RWB + 4 * w + q);
+            s_blk[atomicAdd(s_nblk, 1)] = (uint16_t)(r * RWB + 4 * w + q);
         }
     }
     __syncthreads();
@@ -600,7 +493,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             if (!win) v = aux[(uint32_t)(gy * it.W + gx)];
             return v;
         };
-        const int nblk = s_nblk;
+        const int nblk = *s_nblk;
         uint8_t* p8 = reinterpret_cast<uint8_t*>(s_p32);
         for (int k = t; k < nblk; k += 256) {
             int e = s_blk[k];
@@ -634,68 +527,311 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
             }
         }
     }
+}
+
+// The resolved rows y0 - 1 .. y0 + RTY of a tile's window, read from the
+// resolved graph written by kf_smooth (words laid out as in s_g32).
+__device__ inline void tile_resolved_load(const DevItem& it, const uint8_t* __restrict__ gfin, int x0, int y0,
+                                          uint32_t* s_g32, int t)
+{
+    const uint8_t* g = gfin + it.g_off;
+    if (it.graph_w4) {
+        if (t < GY * RWW) {
+            int r = t / RWW, w = t % RWW;
+            int x = x0 - 8 + 4 * w, y = y0 - 1 + r;
+            uint32_t v = 0;
+            if (w >= 1 && w <= 10 && (unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
+                v = *reinterpret_cast<const uint32_t*>(g + (uint32_t)(y * it.W + x));
+            s_g32[t] = v;
+        }
+    } else {
+        uint8_t* s8 = reinterpret_cast<uint8_t*>(s_g32);
+        for (int k = t; k < GY * RWB; k += 256) {
+            int r = k / RWB, c = k % RWB;
+            int x = x0 - 8 + c, y = y0 - 1 + r;
+            bool ok = c >= 4 && c < 44 && (unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H;
+            s8[k] = ok ? g[(uint32_t)(y * it.W + x)] : 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kf_smooth: crossing resolution, cells and corner-cutting subdivision of a
+// 32x8-cell tile (subdivision_Kernel kernel.cu:216-261 on the cells_Kernel
+// hulls, then triangulate_Kernel's ear clipping, kernel.cu:264-282), each
+// own cell once.  Writes the tile's resolved graph bytes to gfin and, for
+// every smoothed (non-internal) cell, the coverage rows of its triangles at
+// scale S with their summary to `rows` (rows_words(S) words per cell).
+// Internal cells (node 90) keep their hull (kernel.cu:231): the raster takes
+// their rows from the table.  The cells that need smoothing are listed first
+// so that every lane of a wave works on one; each lane's polygon lives in
+// LDS (a column of s_poly), where ear clipping scans and shifts it.
+// ---------------------------------------------------------------------------
+template <int S>
+__global__ __launch_bounds__(256) void kf_smooth(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
+                                                 int grid3d, const uint8_t* __restrict__ in,
+                                                 const uint8_t* __restrict__ gaux, uint8_t* __restrict__ gfin,
+                                                 const int8_t* __restrict__ hull_table, uint32_t* __restrict__ rows)
+{
+    constexpr int MW = mask_w(S), KMIN = mask_kmin(S), RW = rows_words(S);
+    constexpr int HB = 17;  // LDS bytes per hull: n, x[8], y[8] (quarter units)
+    __shared__ uint32_t s_aux32[AY * RWW];
+    __shared__ uint32_t s_p32[(GY + 1) * RWW];
+    __shared__ uint16_t s_blk[(GY + 1) * 40];
+    __shared__ int s_nblk;
+    __shared__ uint32_t s_g32[GY * RWW];
+    __shared__ int8_t s_hull[RCX * RCY * HB];
+    __shared__ uint16_t s_list[RTX * RTY];
+    __shared__ int s_nlist;
+    __shared__ uint32_t s_poly[2 * kMaxPts * 256];  // point k of lane t at [k * 256 + t]: x | y << 16 (1/64 units)
+    const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
+
+    const int t = threadIdx.x;
+    const TileAt at = locate_tile<8>(tile_map, grid3d);
+    const DevItem it = items[at.item];
+    const int x0 = at.tx * RTX, y0 = at.ty * RTY;
+    const uint8_t* aux = gaux + it.g_off;
+    const int ay0 = y0 - 2;
+    if (t == 0) {
+        s_nlist = 0;
+        s_nblk = 0;
+    }
+    if (t < (GY + 1) * RWW) s_p32[t] = 0;
+    tile_snapshot_load(it, aux, x0, ay0, s_aux32, t);
     __syncthreads();
+    tile_resolve(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, t);
+    __syncthreads();
+    // hulls of the own cells and of their neighbours: cell (x0 - 1 + lx,
+    // y0 - 1 + ly) at resolved row ly, byte lx + 7
+    for (int k = t; k < RCX * RCY; k += 256) {
+        const int lx = k % RCX, ly = k / RCX;
+        const int c = ly * RWB + lx + 7;
+        const uint32_t key = cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
+        const int8_t* e = hull_table + (size_t)key * kHullBytes;
+#pragma unroll
+        for (int q = 0; q < HB; q++) s_hull[k * HB + q] = e[q];
+    }
+    // own cells to smooth: in the frame and not internal
+    {
+        const int cx = t % RTX, cy = t / RTX;
+        const uint32_t node = s_g[(cy + 1) * RWB + cx + 8];
+        if (x0 + cx < it.W && y0 + cy < it.H && node != 90) s_list[atomicAdd(&s_nlist, 1)] = (uint16_t)t;
+    }
+    __syncthreads();
+    if (t >= s_nlist) return;
+    const int own = s_list[t];
+    const int cx = own % RTX, cy = own / RTX;
+    const int x = x0 + cx, y = y0 + cy;
+    const uint32_t node = s_g[(cy + 1) * RWB + cx + 8];
+    const int hk = (cy + 1) * RCX + cx + 1;
+    const int8_t* h = s_hull + hk * HB;
+    const int n = h[0];
+    struct HullX {
+        const int8_t* p;
+        __device__ int operator[](int k) const { return p[1 + k]; }
+    };
+    struct HullY {
+        const int8_t* p;
+        __device__ int operator[](int k) const { return p[9 + k]; }
+    };
+    ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags,
+                (it.flags & DPX_INDEXED) ? in + it.pal_off : nullptr };
+    const TjView tv = tj_view(im);
+    auto nbr = [&](int e, auto f) {
+        const int8_t* q = s_hull + (hk + edy(e) * RCX + edx(e)) * HB;
+        return f(HullX{ q }, HullY{ q }, (int)q[0]);
+    };
+    auto tj = [&](int x4, int y4) -> bool { return tjunction(tv, x, y, x4, y4); };
+    uint32_t* P = s_poly + t;  // this lane's column
+    int m = 0;
+    subdivide_cell_emit(HullX{ h }, HullY{ h }, n, node, nbr, tj, [&](int px, int py) {
+        P[256 * m] = ((uint32_t)px & 0xFFFFu) | ((uint32_t)py << 16);
+        m++;
+    });
+    auto px_of = [](uint32_t p) { return (int)(int16_t)(p & 0xFFFFu); };
+    auto py_of = [](uint32_t p) { return (int)p >> 16; };
+    // Ear clipping (process, triangulate_functions.cu:108-217) as ear_clip
+    // does it: vertices in counter-clockwise order (reversed when the
+    // signed area is not positive), first ear tried at (m-1, 0, 1), an ear
+    // needs a positive cross and no other vertex inside or on it, a failure
+    // counter of 2 nv.  Each ear's coverage rows are accumulated as it is cut.
+    uint32_t cov[MW];
+#pragma unroll
+    for (int r = 0; r < MW; r++) cov[r] = 0;
+    if (m >= 3) {
+        int a2 = 0;
+        for (int p = m - 1, q = 0; q < m; p = q++) {
+            const uint32_t A = P[256 * p], B = P[256 * q];
+            a2 += px_of(A) * py_of(B) - px_of(B) * py_of(A);
+        }
+        if (a2 <= 0)
+            for (int lo = 0, hi = m - 1; lo < hi; lo++, hi--) {
+                const uint32_t tmp = P[256 * lo];
+                P[256 * lo] = P[256 * hi];
+                P[256 * hi] = tmp;
+            }
+        int nv = m, count = 2 * nv;
+        for (int v = nv - 1; nv > 2;) {
+            if (0 >= (count--)) break;  // the reference gives up; the triangles cut so far stand
+            int u = v;
+            if (nv <= u) u = 0;
+            v = u + 1;
+            if (nv <= v) v = 0;
+            int w = v + 1;
+            if (nv <= w) w = 0;
+            const uint32_t A = P[256 * u], B = P[256 * v], Cc = P[256 * w];
+            const int ax = px_of(A), ay = py_of(A), bx = px_of(B), by = py_of(B), cx_ = px_of(Cc), cy_ = py_of(Cc);
+            bool ear = ((bx - ax) * (cy_ - ay) - (by - ay) * (cx_ - ax)) > 0;
+            if (ear) {
+                for (int p = 0; p < nv; p++) {
+                    if (p == u || p == v || p == w) continue;
+                    const uint32_t Q = P[256 * p];
+                    if (inside_tri(ax, ay, bx, by, cx_, cy_, px_of(Q), py_of(Q))) {
+                        ear = false;
+                        break;
+                    }
+                }
+            }
+            if (ear) {
+                // coverage of triangle (A, B, C) in units of 1/(128 S) of a
+                // cell; it is counter-clockwise (positive cross)
+                const int tax = ax * 2 * S, tay = ay * 2 * S, tbx = bx * 2 * S, tby = by * 2 * S;
+                const int tcx = cx_ * 2 * S, tcy = cy_ * 2 * S;
+                const int ymin = min(tay, min(tby, tcy)), ymax = max(tay, max(tby, tcy));
+#pragma unroll
+                for (int r = 0; r < MW; r++) {
+                    const int sy = (2 * (KMIN + r) + 1) * 64;
+                    if (sy < ymin || sy > ymax) continue;
+                    cov[r] |= tri_row_bits(tax, tay, tbx, tby, tcx, tcy, sy, KMIN, MW);
+                }
+                for (int s2 = v, t2 = v + 1; t2 < nv; s2++, t2++) P[256 * s2] = P[256 * t2];
+                nv--;
+                count = 2 * nv;
+            }
+        }
+    }
+    uint32_t* dst = rows + it.rows_off / 4 + (size_t)((uint32_t)y * (uint32_t)it.W + (uint32_t)x) * RW;
+#pragma unroll
+    for (int r = 0; r < MW; r++) dst[r] = cov[r];
+    dst[MW] = coverage_info(cov, S);
+}
+
+// `items` are the items of one (scale, subdivision) group, in launch order.
+// With SUB the resolved graph comes from gfin and the coverage rows of the
+// smoothed cells from `cell_rows` (kf_smooth ran before); without, the kernel
+// resolves the crossings itself from the snapshot.
+template <int S, bool SUB>
+__global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
+                                                 int grid3d,
+                                                 const uint8_t* __restrict__ in, const uint8_t* __restrict__ gaux,
+                                                 uint8_t* __restrict__ gfin, const uint32_t* __restrict__ cell_rows,
+                                                 const uint32_t* __restrict__ mask_table,
+                                                 const uint16_t* __restrict__ info_table, uint8_t* __restrict__ out)
+{
+    constexpr int MW = mask_w(S), KMIN = mask_kmin(S), RW = rows_words(S);
+    __shared__ uint32_t s_aux32[SUB ? 1 : AY * RWW];
+    __shared__ uint32_t s_p32[SUB ? 1 : (GY + 1) * RWW];  // decisions of crossed blocks, by origin
+    __shared__ uint16_t s_blk[SUB ? 1 : (GY + 1) * 40];   // crossed blocks: origin byte index in the window
+    __shared__ int s_nblk;
+    __shared__ uint32_t s_g32[GY * RWW];
+    __shared__ uint16_t s_key[RCX * RCY];
+    __shared__ uint8_t s_info[RCX * RCY];  // coverage summary bits 0..4 (simple_cell)
+    __shared__ uint32_t s_col[RCX * RCY];
+    __shared__ uint32_t s_src[SUB ? RCX * RCY : 1];  // smoothed candidates: word offset of their rows
+    __shared__ uint16_t s_list[RTX * RTY];  // own cells that need the candidate search
+    __shared__ int s_nlist;
+    // resolved pixels of the first listed cells, so that every own
+    // cell's rows are stored together (a line is then written by one store
+    // instruction instead of being split between the direct and searched
+    // cells).  A pixel is kept as its colour; from S = 6 a tile whose listed
+    // cells would overflow that keeps the code of each pixel's winning
+    // candidate instead (1 byte: (1 - dy) * 3 + (1 - dx), 9 = clear colour),
+    // four times as many cells.  Measured: codes cost a decode per pixel, so
+    // tiles that fit keep colours; at S = 5 codes lose to the overflow stores.
+    constexpr bool CODES = S >= 6;
+    constexpr int PX_WORDS = 2048, PX_CELLS = PX_WORDS / (S * S), PX_CODE_CELLS = 4 * PX_WORDS / (S * S);
+    static_assert(PX_CELLS >= 1, "at least one buffered cell");
+    __shared__ uint32_t s_px[PX_WORDS];
+    uint8_t* s_px8 = reinterpret_cast<uint8_t*>(s_px);
+    __shared__ uint16_t s_idx[RTX * RTY];    // list index of a listed own cell
+    __shared__ uint8_t s_simple[RTX * RTY];  // own cell shows its own colour only
+    const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
+
+    const int t = threadIdx.x;
+    const TileAt at = locate_tile<8>(tile_map, grid3d);
+    const DevItem it = items[at.item];
+    const int x0 = at.tx * RTX, y0 = at.ty * RTY;
+    const uint8_t* aux = gaux + it.g_off;
+    const int ay0 = y0 - 2;  // first snapshot row
+
+    if (t == 0) {
+        s_nlist = 0;
+        s_nblk = 0;
+    }
+    // The snapshot bytes (or the resolved ones) and the candidates' colours
+    // are independent loads: issue them together.
+    if constexpr (SUB) {
+        tile_resolved_load(it, gfin, x0, y0, s_g32, t);
+    } else {
+        if (t < (GY + 1) * RWW) s_p32[t] = 0;
+        tile_snapshot_load(it, aux, x0, ay0, s_aux32, t);
+    }
+    {
+        const uint8_t* base = in + it.in_off;
+        const uint32_t o0 = (it.flags & DPX_RGB) ? 2 : 0, o2 = 2 - o0;
+        const bool topdown = it.flags & DPX_TOPDOWN, four = it.ch == 4;
+        const bool indexed = it.flags & DPX_INDEXED;
+        const uint32_t* pal = reinterpret_cast<const uint32_t*>(in + it.pal_off);
+        for (int k = t; k < RCX * RCY; k += 256) {
+            int x = x0 - 1 + k % RCX, y = y0 - 1 + k / RCX;
+            if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
+                int row = topdown ? it.H - 1 - y : y;
+                // row * stride < 2^31 (checked on the host); the stride itself may exceed 2^24
+                uint32_t off = (uint32_t)row * (uint32_t)it.stride + __umul24(x, it.ch);
+                if (indexed) {
+                    const uint32_t e = pal[base[off]];
+                    s_col[k] = ((e >> (8 * o2)) & 0xFFu) | (e & 0xFF00u) | ((e >> (8 * o0)) & 0xFFu) << 16 |
+                               (e & 0xFF000000u);
+                } else {
+                    uint32_t a = four ? base[off + 3] : 255u;
+                    s_col[k] = (uint32_t)base[off + o2] | (uint32_t)base[off + 1] << 8 |
+                               (uint32_t)base[off + o0] << 16 | a << 24;  // colorPos (kernel.cu:98-101)
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if constexpr (!SUB) {
+        tile_resolve(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, t);
+        __syncthreads();
+    }
     // candidate cells: key and coverage summary.  Candidate (lx, ly) is the
-    // cell at (x0 - 1 + lx, y0 - 1 + ly): resolved row ly - 1 + GHY, byte lx + 7.
+    // cell at (x0 - 1 + lx, y0 - 1 + ly): resolved row ly, byte lx + 7.
     for (int k = t; k < RCX * RCY; k += 256) {
         int lx = k % RCX, ly = k / RCX;
         int x = x0 - 1 + lx, y = y0 - 1 + ly;
         uint16_t key = kNoCell;
         uint8_t info = 0;
         if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
-            int c = (ly - 1 + GHY) * RWB + lx + 7;
+            int c = ly * RWB + lx + 7;
             key = (uint16_t)cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
-            info = (uint8_t)(info_table[key] & 0x1Fu);
             if constexpr (SUB) {
-                const uint32_t* row = mask_table + (size_t)key * MW;
-#pragma unroll
-                for (int r = 0; r < MW; r++) s_mask[k * MW + r] = row[r];
+                // a smoothed cell's rows and summary come from kf_smooth;
+                // an internal one (node 90) keeps its hull: the table's
+                uint32_t src = 0xFFFFFFFFu;
+                if ((key & 0xFFu) != 90) {
+                    src = (uint32_t)(it.rows_off / 4) + ((uint32_t)y * (uint32_t)it.W + (uint32_t)x) * RW;
+                    info = (uint8_t)(cell_rows[src + MW] & 0x1Fu);
+                } else {
+                    info = (uint8_t)(info_table[key] & 0x1Fu);
+                }
+                s_src[k] = src;
+            } else {
+                info = (uint8_t)(info_table[key] & 0x1Fu);
             }
         }
         s_key[k] = key;
         s_info[k] = info;
-    }
-    if constexpr (SUB) {
-        ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags,
-                    (it.flags & DPX_INDEXED) ? in + it.pal_off : nullptr };
-        for (int k = t; k < RHX * RHY; k += 256) {  // hull (lx, ly): cell (x0 - 2 + lx, y0 - 2 + ly)
-            int lx = k % RHX, ly = k / RHX;
-            int c = (ly - 2 + GHY) * RWB + lx + 6;
-            uint32_t key = cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
-            const int8_t* e = hull_table + (size_t)key * kHullBytes;
-#pragma unroll
-            for (int q = 0; q < 17; q++) s_hull[k * 17 + q] = e[q];
-        }
-        __syncthreads();
-        // smoothed polygons of the non-internal candidate cells
-        TjView tv = tj_view(im);
-        for (int k = t; k < RCX * RCY; k += 256) {
-            if (s_key[k] == kNoCell) continue;
-            int lx = k % RCX, ly = k / RCX;
-            int x = x0 - 1 + lx, y = y0 - 1 + ly;
-            uint32_t node = s_g[(ly - 1 + GHY) * RWB + lx + 7];
-            if (node == 90) continue;  // internal node keeps its hull (kernel.cu:231)
-            int hk = (ly + 1) * RHX + lx + 1;
-            int n = s_hull[hk * 17];
-            int hx[2 * kMaxPts], hy[2 * kMaxPts];
-            for (int q = 0; q < n; q++) { hx[q] = s_hull[hk * 17 + 1 + q]; hy[q] = s_hull[hk * 17 + 9 + q]; }
-            auto nbr = [&](int e, int* lxs, int* lys) -> int {
-                int q = hk + edy(e) * RHX + edx(e);
-                int ln = s_hull[q * 17];
-                for (int u = 0; u < ln; u++) { lxs[u] = s_hull[q * 17 + 1 + u]; lys[u] = s_hull[q * 17 + 9 + u]; }
-                return ln;
-            };
-            auto tj = [&](int x4, int y4) -> bool { return tjunction(tv, x, y, x4, y4); };
-            int ox[kMaxPoly], oy[kMaxPoly];
-            int m = subdivide_cell(hx, hy, n, node, nbr, tj, ox, oy);
-            uint8_t tri[3 * kMaxPoly];
-            int nt = ear_clip_compact(ox, oy, m, tri);
-            uint32_t rows[MW];
-            coverage_rows<S>(ox, oy, tri, nt, rows);
-#pragma unroll
-            for (int r = 0; r < MW; r++) s_mask[k * MW + r] = rows[r];
-            s_info[k] = (uint8_t)(coverage_info(rows, S) & 0x1Fu);
-        }
     }
     __syncthreads();
 
@@ -770,8 +906,12 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
                 uint32_t row = 0;
                 const bool before_own = dy < 0 || (dy == 0 && dx < 0);
                 if (!(own_full && before_own) && r >= 0 && r < MW && s_key[k] != kNoCell) {
-                    if constexpr (SUB) row = s_mask[k * MW + r];
-                    else row = mask_table[(uint32_t)s_key[k] * MW + r];
+                    if constexpr (SUB) {
+                        const uint32_t src = s_src[k];
+                        row = src != 0xFFFFFFFFu ? cell_rows[src + r] : mask_table[(uint32_t)s_key[k] * MW + r];
+                    } else {
+                        row = mask_table[(uint32_t)s_key[k] * MW + r];
+                    }
                 }
                 rows[(1 - dy) * 3 + (1 - dx)] = row;
             }
@@ -1062,7 +1202,9 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
 }
 
 using RasterFn = void (*)(const DevItem*, const uint2*, int, const uint8_t*, const uint8_t*, uint8_t*,
-                          const int8_t*, const uint32_t*, const uint16_t*, uint8_t*);
+                          const uint32_t*, const uint32_t*, const uint16_t*, uint8_t*);
+using SmoothFn = void (*)(const DevItem*, const uint2*, int, const uint8_t*, const uint8_t*, uint8_t*,
+                          const int8_t*, uint32_t*);
 
 // The raster kernel of scale S (1..15) with (SUB) or without subdivision.
 template <bool SUB>
@@ -1088,17 +1230,33 @@ RasterFn pick_raster_of(int S)
     return nullptr;
 }
 
-#ifdef DPX_RASTER_SUB_TU
-RasterFn pick_raster_sub(int S) { return pick_raster_of<true>(S); }
-#else
-RasterFn pick_raster_sub(int S);  // in the DPX_RASTER_SUB_TU build of this file
+RasterFn pick_raster(int S, bool sub) { return sub ? pick_raster_of<true>(S) : pick_raster_of<false>(S); }
 
-RasterFn pick_raster(int S, bool sub) { return sub ? pick_raster_sub(S) : pick_raster_of<false>(S); }
-#endif
+// The smoothing kernel of scale S (1..15).
+SmoothFn pick_smooth(int S)
+{
+    switch (S) {
+        case 1: return kf_smooth<1>;
+        case 2: return kf_smooth<2>;
+        case 3: return kf_smooth<3>;
+        case 4: return kf_smooth<4>;
+        case 5: return kf_smooth<5>;
+        case 6: return kf_smooth<6>;
+        case 7: return kf_smooth<7>;
+        case 8: return kf_smooth<8>;
+        case 9: return kf_smooth<9>;
+        case 10: return kf_smooth<10>;
+        case 11: return kf_smooth<11>;
+        case 12: return kf_smooth<12>;
+        case 13: return kf_smooth<13>;
+        case 14: return kf_smooth<14>;
+        case 15: return kf_smooth<15>;
+    }
+    return nullptr;
+}
 
 }  // namespace dpx
 
-#ifndef DPX_RASTER_SUB_TU
 
 // ---------------------------------------------------------------------------
 // context
@@ -1132,6 +1290,8 @@ struct dpx_context {
     uint16_t* d_info[dpx::kMaxScale + 1] = {};
     uint8_t *d_gaux = nullptr, *d_gfin = nullptr;
     size_t g_cap = 0;
+    uint32_t* d_rows = nullptr;  // cell rows of smoothed cells (kf_smooth)
+    size_t rows_cap = 0;
     uint8_t* d_desc = nullptr;
     size_t desc_cap = 0;
     uint8_t* h_pinned = nullptr;
@@ -1196,6 +1356,19 @@ int grow_graph(dpx_context* ctx, size_t need)
     DPX_HIP(hipMalloc(&ctx->d_gaux, cap));
     DPX_HIP(hipMalloc(&ctx->d_gfin, cap));
     ctx->g_cap = cap;
+    return DPX_OK;
+}
+
+int grow_rows(dpx_context* ctx, size_t need)
+{
+    if (need <= ctx->rows_cap) return DPX_OK;
+    DPX_HIP(hipDeviceSynchronize());
+    (void)hipFree(ctx->d_rows);
+    ctx->d_rows = nullptr;
+    ctx->rows_cap = 0;
+    size_t cap = align_up(need + need / 4, 1 << 20);
+    DPX_HIP(hipMalloc(&ctx->d_rows, cap));
+    ctx->rows_cap = cap;
     return DPX_OK;
 }
 
@@ -1290,6 +1463,7 @@ extern "C" void dpx_context_destroy(dpx_context* ctx)
     for (auto* m : ctx->d_info) (void)hipFree(m);
     (void)hipFree(ctx->d_gaux);
     (void)hipFree(ctx->d_gfin);
+    (void)hipFree(ctx->d_rows);
     (void)hipFree(ctx->d_desc);
     if (ctx->h_pinned) (void)hipHostFree(ctx->h_pinned);
     if (ctx->upload_done) (void)hipEventDestroy(ctx->upload_done);
@@ -1310,6 +1484,15 @@ extern "C" int dpx_context_reserve(dpx_context* ctx, int max_items, int64_t max_
     return DPX_OK;
 }
 
+namespace dpx {
+int context_reserve_rows(dpx_context* ctx, size_t bytes)
+{
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    DPX_HIP(hipSetDevice(ctx->device));
+    return grow_rows(ctx, bytes);
+}
+}  // namespace dpx
+
 extern "C" int dpx_context_keep_graph(dpx_context* ctx, int enable)
 {
     DPX_CHECK(ctx, DPX_EINVAL, "null context");
@@ -1327,55 +1510,68 @@ extern "C" int dpx_batch_graph(dpx_context* ctx, const uint8_t** d_graph, const 
     return DPX_OK;
 }
 
+// Validates item k and fills its device descriptor; its graph bytes start
+// at g_off of the graph scratch.
+static int make_dev_item(const dpx_item& a, int k, const uint8_t* d_out, uint64_t g_off, DevItem& d)
+{
+    using namespace dpx;
+    DPX_CHECK(a.width > 0 && a.height > 0 && a.width <= 32768 && a.height <= 32768, DPX_EINVAL,
+              "item %d: bad size %dx%d", k, a.width, a.height);
+    DPX_CHECK((a.flags & ~kKnownFlags) == 0, DPX_EINVAL, "item %d: unknown flags 0x%x", k, a.flags & ~kKnownFlags);
+    const bool indexed = a.flags & DPX_INDEXED;
+    DPX_CHECK(indexed ? a.channels == 1 : (a.channels == 3 || a.channels == 4), DPX_EINVAL, "item %d: channels %d%s",
+              k, a.channels, indexed ? " (indexed frames have 1)" : "");
+    DPX_CHECK(!indexed || a.palette_offset % 4 == 0, DPX_EINVAL, "item %d: palette offset %llu not a multiple of 4", k,
+              (unsigned long long)a.palette_offset);
+    DPX_CHECK(a.stride >= a.width * a.channels, DPX_EINVAL, "item %d: stride %d", k, a.stride);
+    DPX_CHECK((int64_t)a.stride * a.height < (int64_t)1 << 31, DPX_EINVAL, "item %d: input larger than 2 GiB", k);
+    DPX_CHECK(a.scale >= 1 && a.scale <= kMaxScale, DPX_EINVAL, "item %d: scale %d outside 1..%d", k, a.scale,
+              kMaxScale);
+    DPX_CHECK(a.out_stride >= 4 * a.width * a.scale && a.out_stride % 4 == 0 && a.out_offset % 4 == 0, DPX_EINVAL,
+              "item %d: output layout (stride %d, offset %llu)", k, a.out_stride, (unsigned long long)a.out_offset);
+    DPX_CHECK(!a.subdivide || (uint64_t)a.width * a.height * rows_words(a.scale) < ((uint64_t)1 << 32), DPX_EINVAL,
+              "item %d: %dx%d at scale %d with subdivision exceeds 2^32 cell-row words", k, a.width, a.height, a.scale);
+    memset(&d, 0, sizeof(d));
+    d.in_off = a.in_offset;
+    d.out_off = a.out_offset;
+    d.g_off = g_off;
+    d.W = a.width;
+    d.H = a.height;
+    d.stride = a.stride;
+    d.ch = a.channels;
+    d.flags = a.flags;
+    d.pal_off = indexed ? a.palette_offset : 0;
+    d.out_stride = a.out_stride;
+    d.scale = a.scale;
+    d.subdivide = a.subdivide ? 1 : 0;
+    d.aligned16 = (a.out_stride % 16 == 0) && (a.out_offset % 16 == 0) && (reinterpret_cast<uintptr_t>(d_out) % 16 == 0);
+    d.graph_w4 = a.width % 4 == 0;
+    return DPX_OK;
+}
+
 // Validates the items and builds a launch plan: device items, graph tiles,
 // raster groups by (scale, subdivision), the descriptor blob (offsets in the
 // plan refer to it) and the graph scratch bytes the batch needs.
 static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* items, int n_items, BatchPlan& plan,
-                      std::vector<uint8_t>& blob, std::vector<uint64_t>* g_offsets, size_t* graph_bytes)
+                      std::vector<uint8_t>& blob, std::vector<uint64_t>* g_offsets, size_t* graph_bytes,
+                      size_t* rows_bytes)
 {
     using namespace dpx;
     plan.valid = false;
     std::vector<DevItem> dev(n_items);
     if (g_offsets) g_offsets->assign(n_items, 0);
     std::map<int, Group> groups;
-    uint64_t goff = 0;
+    uint64_t goff = 0, roff = 0;
     for (int k = 0; k < n_items; k++) {
         const dpx_item& a = items[k];
-        DPX_CHECK(a.width > 0 && a.height > 0 && a.width <= 32768 && a.height <= 32768, DPX_EINVAL,
-                  "item %d: bad size %dx%d", k, a.width, a.height);
-        DPX_CHECK((a.flags & ~kKnownFlags) == 0, DPX_EINVAL, "item %d: unknown flags 0x%x", k,
-                  a.flags & ~kKnownFlags);
-        const bool indexed = a.flags & DPX_INDEXED;
-        DPX_CHECK(indexed ? a.channels == 1 : (a.channels == 3 || a.channels == 4), DPX_EINVAL,
-                  "item %d: channels %d%s", k, a.channels, indexed ? " (indexed frames have 1)" : "");
-        DPX_CHECK(!indexed || a.palette_offset % 4 == 0, DPX_EINVAL, "item %d: palette offset %llu not a multiple of 4",
-                  k, (unsigned long long)a.palette_offset);
-        DPX_CHECK(a.stride >= a.width * a.channels, DPX_EINVAL, "item %d: stride %d", k, a.stride);
-        DPX_CHECK((int64_t)a.stride * a.height < (int64_t)1 << 31, DPX_EINVAL, "item %d: input larger than 2 GiB", k);
-        DPX_CHECK(a.scale >= 1 && a.scale <= kMaxScale, DPX_EINVAL, "item %d: scale %d outside 1..%d", k, a.scale,
-                  kMaxScale);
-        DPX_CHECK(a.out_stride >= 4 * a.width * a.scale && a.out_stride % 4 == 0 && a.out_offset % 4 == 0, DPX_EINVAL,
-                  "item %d: output layout (stride %d, offset %llu)", k, a.out_stride,
-                  (unsigned long long)a.out_offset);
         DevItem& d = dev[k];
-        memset(&d, 0, sizeof(d));
-        d.in_off = a.in_offset;
-        d.out_off = a.out_offset;
-        d.g_off = goff;
-        d.W = a.width;
-        d.H = a.height;
-        d.stride = a.stride;
-        d.ch = a.channels;
-        d.flags = a.flags;
-        d.pal_off = indexed ? a.palette_offset : 0;
-        d.out_stride = a.out_stride;
-        d.scale = a.scale;
-        d.subdivide = a.subdivide ? 1 : 0;
-        d.aligned16 = (a.out_stride % 16 == 0) && (a.out_offset % 16 == 0) &&
-                      (reinterpret_cast<uintptr_t>(d_out) % 16 == 0);
-        d.graph_w4 = a.width % 4 == 0;
+        if (int rc = make_dev_item(a, k, d_out, goff, d)) return rc;
         if (g_offsets) (*g_offsets)[k] = goff;
         goff += align_up((size_t)a.width * a.height, 256);
+        if (d.subdivide) {
+            d.rows_off = roff;
+            roff += align_up((size_t)a.width * a.height * rows_words(a.scale) * 4, 256);
+        }
         int gkey = a.scale * 2 + d.subdivide;
         Group& grp = groups[gkey];
         grp.scale = a.scale;
@@ -1413,6 +1609,7 @@ static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* it
     plan.items.assign(items, items + n_items);
     plan.out_mod = reinterpret_cast<uintptr_t>(d_out) % 16;
     *graph_bytes = goff;
+    *rows_bytes = roff;
     plan.valid = true;
     return DPX_OK;
 }
@@ -1424,11 +1621,12 @@ static int make_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* ite
 {
     BatchPlan& plan = ctx->plan;
     std::vector<uint8_t> blob;
-    size_t gbytes = 0;
-    if (int rc = build_plan(ctx, d_out, items, n_items, plan, blob, &ctx->g_offsets, &gbytes)) return rc;
+    size_t gbytes = 0, rbytes = 0;
+    if (int rc = build_plan(ctx, d_out, items, n_items, plan, blob, &ctx->g_offsets, &gbytes, &rbytes)) return rc;
     plan.valid = false;
-    // graph arenas (grown only when a batch needs more)
+    // graph and cell-row arenas (grown only when a batch needs more)
     if (int rc = grow_graph(ctx, gbytes)) return rc;
+    if (int rc = grow_rows(ctx, rbytes)) return rc;
     if (blob != ctx->last_desc) {
         if (ctx->upload_pending) {
             DPX_HIP(hipEventSynchronize(ctx->upload_done));
@@ -1450,7 +1648,9 @@ static int make_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* ite
 struct LaunchArgs {
     const uint8_t* desc;
     uint8_t* gaux;
-    uint8_t* gfin;  // null: the resolved graph is not written
+    uint8_t* gfin;  // the resolved graph (written by kf_smooth for subdivided items)
+    bool keep;      // the raster of the other items writes the resolved graph too
+    uint32_t* rows;  // cell rows of smoothed cells
 };
 
 static int launch_plan(dpx_context* ctx, const BatchPlan& plan, const LaunchArgs& la, const uint8_t* d_in,
@@ -1465,8 +1665,15 @@ static int launch_plan(dpx_context* ctx, const BatchPlan& plan, const LaunchArgs
         DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", pg.scale);
         const DevItem* d_gitems = reinterpret_cast<const DevItem*>(la.desc + pg.off_items);
         const uint2* d_rmap = reinterpret_cast<const uint2*>(la.desc + pg.off_map);
-        hipLaunchKernelGGL(fn, pg.grid, dim3(256), 0, stream, d_gitems, d_rmap, pg.grid3d, d_in, la.gaux, la.gfin,
-                           ctx->d_hull, ctx->d_mask[pg.scale], ctx->d_info[pg.scale], d_out);
+        if (pg.sub) {
+            SmoothFn sf = pick_smooth(pg.scale);
+            DPX_CHECK(sf && la.gfin && la.rows, DPX_EINVAL, "no smoothing buffers for scale %d", pg.scale);
+            hipLaunchKernelGGL(sf, pg.grid, dim3(256), 0, stream, d_gitems, d_rmap, pg.grid3d, d_in, la.gaux, la.gfin,
+                               ctx->d_hull, la.rows);
+        }
+        hipLaunchKernelGGL(fn, pg.grid, dim3(256), 0, stream, d_gitems, d_rmap, pg.grid3d, d_in, la.gaux,
+                           (pg.sub || la.keep) ? la.gfin : nullptr, la.rows, ctx->d_mask[pg.scale],
+                           ctx->d_info[pg.scale], d_out);
     }
     DPX_HIP(hipGetLastError());
     return DPX_OK;
@@ -1496,7 +1703,7 @@ extern "C" int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_o
         ctx->ev_used += 3;
         DPX_HIP(hipEventRecord(ev[0], stream));
     }
-    LaunchArgs la{ ctx->d_desc, ctx->d_gaux, ctx->keep_graph ? ctx->d_gfin : nullptr };
+    LaunchArgs la{ ctx->d_desc, ctx->d_gaux, ctx->d_gfin, ctx->keep_graph, ctx->d_rows };
     if (int rc = launch_plan(ctx, plan, la, d_in, d_out, stream, ev[1])) return rc;
     ctx->last_kept = ctx->keep_graph;
     if (ctx->timing) DPX_HIP(hipEventRecord(ev[2], stream));
@@ -1513,13 +1720,16 @@ struct dpx_plan {
     uint8_t* d_desc = nullptr;
     uint8_t* d_gaux = nullptr;
     uint8_t* d_gfin = nullptr;
+    uint32_t* d_rows = nullptr;
     uint32_t flags = 0;
     std::mutex mu;
-    // DPX_PLAN_GRAPH: the kernels as graph nodes; the input and output
+    // DPX_PLAN_GRAPH: the kernels as graph nodes (kf_graph, then per group
+    // kf_smooth when it subdivides and kf_raster); the input and output
     // pointers of the instantiated graph are updated in place when they change
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     std::vector<hipGraphNode_t> nodes;
+    std::vector<std::pair<int, int>> node_kind;  // (0 graph / 1 smooth / 2 raster, group)
     const uint8_t* g_in = nullptr;
     uint8_t* g_out = nullptr;
     std::vector<uint64_t> g_offsets;  // per item, in d_gfin
@@ -1535,11 +1745,12 @@ void plan_free(dpx_plan* p)
     (void)hipFree(p->d_desc);
     (void)hipFree(p->d_gaux);
     (void)hipFree(p->d_gfin);
+    (void)hipFree(p->d_rows);
     delete p;
 }
 
-// Kernel node parameters of the plan's kernels for (d_in, d_out).  `store`
-// keeps the argument values the parameter pointers refer to.
+// Kernel node parameters of one of the plan's kernels for (d_in, d_out);
+// `a` keeps the argument values the parameter pointers refer to.
 struct NodeArgs {
     const DevItem* items;
     const uint2* map;
@@ -1549,13 +1760,15 @@ struct NodeArgs {
     uint8_t* gaux;
     uint8_t* gfin;
     const int8_t* hull;
+    const uint32_t* rows_c;
+    uint32_t* rows;
     const uint32_t* mask;
     const uint16_t* info;
     uint8_t* out;
     void* ptrs[10];
 };
 
-hipKernelNodeParams node_params(dpx_plan* p, int k, const uint8_t* d_in, uint8_t* d_out, NodeArgs& a)
+hipKernelNodeParams node_params(dpx_plan* p, int kind, int group, const uint8_t* d_in, uint8_t* d_out, NodeArgs& a)
 {
     hipKernelNodeParams np;
     memset(&np, 0, sizeof(np));
@@ -1564,7 +1777,7 @@ hipKernelNodeParams node_params(dpx_plan* p, int k, const uint8_t* d_in, uint8_t
     a.out = d_out;
     a.gaux = p->d_gaux;
     a.gaux_c = p->d_gaux;
-    if (k == 0) {
+    if (kind == 0) {
         a.items = reinterpret_cast<const DevItem*>(p->d_desc + p->plan.off_items);
         a.map = reinterpret_cast<const uint2*>(p->d_desc + p->plan.off_gmap);
         a.g3d = p->plan.g3d;
@@ -1573,17 +1786,25 @@ hipKernelNodeParams node_params(dpx_plan* p, int k, const uint8_t* d_in, uint8_t
         np.func = reinterpret_cast<void*>(kf_graph);
         np.gridDim = p->plan.ggrid;
     } else {
-        const PlanGroup& pg = p->plan.groups[k - 1];
+        const PlanGroup& pg = p->plan.groups[group];
         a.items = reinterpret_cast<const DevItem*>(p->d_desc + pg.off_items);
         a.map = reinterpret_cast<const uint2*>(p->d_desc + pg.off_map);
         a.g3d = pg.grid3d;
-        a.gfin = p->d_gfin;
-        a.hull = p->ctx->d_hull;
-        a.mask = p->ctx->d_mask[pg.scale];
-        a.info = p->ctx->d_info[pg.scale];
-        void* v[10] = { &a.items, &a.map, &a.g3d, &a.in, &a.gaux_c, &a.gfin, &a.hull, &a.mask, &a.info, &a.out };
-        memcpy(a.ptrs, v, sizeof(v));
-        np.func = reinterpret_cast<void*>(pick_raster(pg.scale, pg.sub));
+        a.gfin = (pg.sub || (p->flags & DPX_PLAN_KEEP_GRAPH)) ? p->d_gfin : nullptr;
+        a.rows = p->d_rows;
+        a.rows_c = p->d_rows;
+        if (kind == 1) {
+            a.hull = p->ctx->d_hull;
+            void* v[8] = { &a.items, &a.map, &a.g3d, &a.in, &a.gaux_c, &a.gfin, &a.hull, &a.rows };
+            memcpy(a.ptrs, v, sizeof(v));
+            np.func = reinterpret_cast<void*>(pick_smooth(pg.scale));
+        } else {
+            a.mask = p->ctx->d_mask[pg.scale];
+            a.info = p->ctx->d_info[pg.scale];
+            void* v[10] = { &a.items, &a.map, &a.g3d, &a.in, &a.gaux_c, &a.gfin, &a.rows_c, &a.mask, &a.info, &a.out };
+            memcpy(a.ptrs, v, sizeof(v));
+            np.func = reinterpret_cast<void*>(pick_raster(pg.scale, pg.sub));
+        }
         np.gridDim = pg.grid;
     }
     np.kernelParams = a.ptrs;
@@ -1593,15 +1814,21 @@ hipKernelNodeParams node_params(dpx_plan* p, int k, const uint8_t* d_in, uint8_t
 int plan_build_graph(dpx_plan* p, const uint8_t* d_in, uint8_t* d_out)
 {
     DPX_HIP(hipGraphCreate(&p->graph, 0));
-    const int nk = 1 + (int)p->plan.groups.size();
-    p->nodes.assign(nk, nullptr);
-    for (int k = 0; k < nk; k++) {
+    p->node_kind.assign(1, { 0, 0 });
+    for (int g = 0; g < (int)p->plan.groups.size(); g++) {
+        if (p->plan.groups[g].sub) p->node_kind.push_back({ 1, g });
+        p->node_kind.push_back({ 2, g });
+    }
+    p->nodes.assign(p->node_kind.size(), nullptr);
+    for (size_t k = 0; k < p->nodes.size(); k++) {
         NodeArgs a;
         memset(&a, 0, sizeof(a));
-        hipKernelNodeParams np = node_params(p, k, d_in, d_out, a);
-        DPX_CHECK(np.func, DPX_EINVAL, "no raster kernel for a group of the plan");
-        // every raster node depends on the graph node only
-        hipGraphNode_t dep = p->nodes[0];
+        const int kind = p->node_kind[k].first;
+        hipKernelNodeParams np = node_params(p, kind, p->node_kind[k].second, d_in, d_out, a);
+        DPX_CHECK(np.func, DPX_EINVAL, "no kernel for a group of the plan");
+        // kf_smooth and the rasters of groups without subdivision follow the
+        // graph kernel; a group's raster after subdivision follows its kf_smooth
+        hipGraphNode_t dep = (kind == 2 && k > 0 && p->node_kind[k - 1].first == 1) ? p->nodes[k - 1] : p->nodes[0];
         DPX_HIP(hipGraphAddKernelNode(&p->nodes[k], p->graph, k ? &dep : nullptr, k ? 1 : 0, &np));
     }
     DPX_HIP(hipGraphInstantiate(&p->exec, p->graph, nullptr, nullptr, 0));
@@ -1626,16 +1853,18 @@ extern "C" int dpx_plan_create(dpx_context* ctx, const dpx_item* items, int n_it
     p->ctx = ctx;
     p->flags = flags;
     std::vector<uint8_t> blob;
-    size_t gbytes = 0;
+    size_t gbytes = 0, rbytes = 0;
     // out_align stands for d_out modulo 16 (only the alignment of the output
     // arena enters the plan)
     const uint8_t* fake_out = reinterpret_cast<const uint8_t*>((uintptr_t)(out_align % 16) + 16);
-    int rc = build_plan(ctx, fake_out, items, n_items, p->plan, blob, nullptr, &gbytes);
+    int rc = build_plan(ctx, fake_out, items, n_items, p->plan, blob, nullptr, &gbytes, &rbytes);
+    const bool want_gfin = (flags & DPX_PLAN_KEEP_GRAPH) || rbytes > 0;
     if (rc == DPX_OK && (hipMalloc(&p->d_desc, blob.size()) != hipSuccess ||
                          hipMalloc(&p->d_gaux, std::max<size_t>(gbytes, 256)) != hipSuccess ||
-                         ((flags & DPX_PLAN_KEEP_GRAPH) &&
-                          hipMalloc(&p->d_gfin, std::max<size_t>(gbytes, 256)) != hipSuccess)))
-        rc = fail(DPX_ENOMEM, "plan allocation failed (%zu descriptor bytes, %zu graph bytes)", blob.size(), gbytes);
+                         (want_gfin && hipMalloc(&p->d_gfin, std::max<size_t>(gbytes, 256)) != hipSuccess) ||
+                         (rbytes > 0 && hipMalloc(&p->d_rows, rbytes) != hipSuccess)))
+        rc = fail(DPX_ENOMEM, "plan allocation failed (%zu descriptor, %zu graph and %zu cell-row bytes)", blob.size(),
+                  gbytes, rbytes);
     if (rc == DPX_OK && hipMemcpy(p->d_desc, blob.data(), blob.size(), hipMemcpyHostToDevice) != hipSuccess)
         rc = fail(DPX_EHIP, "descriptor upload failed");
     if (rc != DPX_OK) {
@@ -1656,16 +1885,16 @@ extern "C" int dpx_plan_run(dpx_plan* p, const uint8_t* d_in, uint8_t* d_out, vo
     hipStream_t stream = as_stream(stream_);
     DPX_HIP(hipSetDevice(p->ctx->device));
     if (!(p->flags & DPX_PLAN_GRAPH)) {
-        LaunchArgs la{ p->d_desc, p->d_gaux, p->d_gfin };
+        LaunchArgs la{ p->d_desc, p->d_gaux, p->d_gfin, (p->flags & DPX_PLAN_KEEP_GRAPH) != 0, p->d_rows };
         return launch_plan(p->ctx, p->plan, la, d_in, d_out, stream, nullptr);
     }
     if (!p->exec) {
         if (int rc = plan_build_graph(p, d_in, d_out)) return rc;
     } else if (p->g_in != d_in || p->g_out != d_out) {
-        for (int k = 0; k < (int)p->nodes.size(); k++) {
+        for (size_t k = 0; k < p->nodes.size(); k++) {
             NodeArgs a;
             memset(&a, 0, sizeof(a));
-            hipKernelNodeParams np = node_params(p, k, d_in, d_out, a);
+            hipKernelNodeParams np = node_params(p, p->node_kind[k].first, p->node_kind[k].second, d_in, d_out, a);
             DPX_HIP(hipGraphExecKernelNodeSetParams(p->exec, p->nodes[k], &np));
         }
         p->g_in = d_in;
@@ -1678,7 +1907,7 @@ extern "C" int dpx_plan_run(dpx_plan* p, const uint8_t* d_in, uint8_t* d_out, vo
 extern "C" int dpx_plan_graph(dpx_plan* p, const uint8_t** d_graph, const uint64_t** offsets_host)
 {
     DPX_CHECK(p && d_graph && offsets_host, DPX_EINVAL, "bad arguments");
-    DPX_CHECK(p->d_gfin, DPX_EINVAL, "the plan was made without DPX_PLAN_KEEP_GRAPH");
+    DPX_CHECK(p->flags & DPX_PLAN_KEEP_GRAPH, DPX_EINVAL, "the plan was made without DPX_PLAN_KEEP_GRAPH");
     std::lock_guard<std::mutex> lock(p->mu);
     if (p->g_offsets.empty()) {
         uint64_t off = 0;
@@ -1727,5 +1956,3 @@ extern "C" int dpx_batch_timing_read(dpx_context* ctx, dpx_batch_times* out)
     ctx->acc = dpx_batch_times{};
     return DPX_OK;
 }
-
-#endif  // DPX_RASTER_SUB_TU

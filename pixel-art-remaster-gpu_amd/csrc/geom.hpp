@@ -188,9 +188,13 @@ __host__ __device__ inline int cell_hull(uint32_t key, int* hx, int* hy)
 // ---------------------------------------------------------------------------
 __host__ __device__ inline int mod_n(int i, int n) { return (n + (i % n)) % n; }
 
+// The functions below take a polygon as any indexable type (a pointer to
+// int, or an accessor whose operator[] reads packed storage).
+
 // isLinkedEdge (245-424): link direction of hull edge idx, or -1 (border).
 // slope == +-1 is dy == +-dx, dx != 0; mid_y > 0.5 is y_a + y_b > 2 quarter units * 2.
-__host__ __device__ inline int linked_edge(const int* hx, const int* hy, int n, uint32_t node, int idx)
+template <class HX, class HY>
+__host__ __device__ inline int linked_edge(const HX& hx, const HY& hy, int n, uint32_t node, int idx)
 {
     int b = mod_n(idx + 1, n);
     int ax = hx[idx], ay = hy[idx], bx = hx[b], by = hy[b];
@@ -211,7 +215,8 @@ __host__ __device__ inline int linked_edge(const int* hx, const int* hy, int n, 
 
 // getQ_i / getR_i (42-122) on a quarter-unit polygon, result in 1/64 units.
 // |P_next - P| <= 1  <=>  dx^2 + dy^2 <= 16 (quarter units).
-__host__ __device__ inline void q_point(const int* hx, const int* hy, int n, int i, int* qx, int* qy)
+template <class HX, class HY>
+__host__ __device__ inline void q_point(const HX& hx, const HY& hy, int n, int i, int* qx, int* qy)
 {
     int p = i % n;
     if (p < 0) p += n;
@@ -222,7 +227,8 @@ __host__ __device__ inline void q_point(const int* hx, const int* hy, int n, int
     else                         { *qx = 2 * (7 * x0 + x1); *qy = 2 * (7 * y0 + y1); }
 }
 
-__host__ __device__ inline void r_point(const int* hx, const int* hy, int n, int i, int* rx, int* ry)
+template <class HX, class HY>
+__host__ __device__ inline void r_point(const HX& hx, const HY& hy, int n, int i, int* rx, int* ry)
 {
     int p = i % n;
     if (p < 0) p += n;
@@ -234,66 +240,99 @@ __host__ __device__ inline void r_point(const int* hx, const int* hy, int n, int
 }
 
 // getPointIndex (527-538): index of (x,y) in the polygon, 0 if absent.
-__host__ __device__ inline int point_index(int x, int y, const int* hx, const int* hy, int n)
+template <class HX, class HY>
+__host__ __device__ inline int point_index(int x, int y, const HX& hx, const HY& hy, int n)
 {
     for (int i = 0; i < n; i++)
         if (hx[i] == x && hy[i] == y) return i;
     return 0;
 }
 
-// subdivision (subdivision_functions.cu:564-669) of one cell.
-//   hx,hy,n   the cell's hull (quarter units, the snapshot of kernel.cu:434)
+// subdivision (subdivision_functions.cu:564-669) of one cell, emitting the
+// new polygon's points in order.
+//   hx,hy,n   the cell's hull (quarter units, the snapshot of kernel.cu:434),
+//             n <= 16
 //   node      the cell's resolved graph byte
-//   nbr(e, lx, ly) -> ln   hull of the neighbour across graph edge e (snapshot)
+//   nbr(e, f) calls f(lx, ly, ln) with the hull of the neighbour across
+//             graph edge e (snapshot) and returns what f returns
 //   tjunction(x4, y4) -> bool   checkTJunction (170-242) for hull vertex (x4,y4)
-// Writes the new polygon in 1/64 units to ox,oy and returns its size.
-template <class NbrHull, class TJunction>
-__host__ __device__ inline int subdivide_cell(const int* hx, const int* hy, int n, uint32_t node, NbrHull nbr,
-                                              TJunction tjunction, int* ox, int* oy)
+//   emit(x, y)  receives each point of the new polygon, in 1/64 units
+// Emits at most 2n points.
+template <class HX, class HY, class NbrHull, class TJunction, class Emit>
+__host__ __device__ inline void subdivide_cell_emit(const HX& hx, const HY& hy, int n, uint32_t node, NbrHull nbr,
+                                                    TJunction tjunction, Emit emit)
 {
-    int link[kMaxPoly];
-    for (int e = 0; e < n; e++) link[e] = linked_edge(hx, hy, n, node, e);
-    int lx[2 * kMaxPts], ly[2 * kMaxPts];
-    int out = 0;
+    // link direction of every edge, 4 bits each (15 = border)
+    uint64_t links = 0;
+    for (int e = 0; e < n; e++) links |= (uint64_t)(linked_edge(hx, hy, n, node, e) & 15) << (4 * e);
+    auto link = [&](int e) { return (int)((links >> (4 * e)) & 15u); };
     for (int k = 0; k < n; k++) {
         int prev = mod_n(k - 1, n);
-        bool sk = link[k] >= 0, sp = link[prev] >= 0;
+        const int lk = link(k), lp = link(prev);
+        bool sk = lk != 15, sp = lp != 15;
         if (!sk && !sp) {
             // two border edges: keep a T-junction corner, else cut it (R_{k-1}, Q_k)
             if (tjunction(hx[k], hy[k])) {
-                ox[out] = 16 * hx[k]; oy[out] = 16 * hy[k]; out++;
+                emit(16 * hx[k], 16 * hy[k]);
             } else {
-                r_point(hx, hy, n, k - 1, &ox[out], &oy[out]); out++;
-                q_point(hx, hy, n, k, &ox[out], &oy[out]); out++;
+                int rx, ry, qx, qy;
+                r_point(hx, hy, n, k - 1, &rx, &ry);
+                q_point(hx, hy, n, k, &qx, &qy);
+                emit(rx, ry);
+                emit(qx, qy);
             }
         } else if (!sk && sp) {
             // edge k border, edge k-1 linked: blend with the neighbour's R
             int qx, qy;
             q_point(hx, hy, n, k, &qx, &qy);
-            int e = link[prev];
-            int ln = nbr(e, lx, ly);
-            int op = point_index(hx[k] - 4 * edx(e), hy[k] - 4 * edy(e), lx, ly, ln);
-            int rx, ry;
-            r_point(lx, ly, ln, op - 1, &rx, &ry);
+            const int e = lp;
+            int rx = 0, ry = 0;
+            nbr(e, [&](const auto& lx, const auto& ly, int ln) {
+                int op = point_index(hx[k] - 4 * edx(e), hy[k] - 4 * edy(e), lx, ly, ln);
+                r_point(lx, ly, ln, op - 1, &rx, &ry);
+                return 0;
+            });
             rx += 64 * edx(e); ry += 64 * edy(e);
-            ox[out] = (qx + rx) / 2; oy[out] = (qy + ry) / 2; out++;
-            ox[out] = qx; oy[out] = qy; out++;
+            emit((qx + rx) / 2, (qy + ry) / 2);
+            emit(qx, qy);
         } else if (sk && !sp) {
             // edge k linked, edge k-1 border: blend with the neighbour's Q
-            int e = link[k];
-            int ln = nbr(e, lx, ly);
-            int op = point_index(hx[k] - 4 * edx(e), hy[k] - 4 * edy(e), lx, ly, ln);
-            int qx, qy;
-            q_point(lx, ly, ln, op, &qx, &qy);
+            const int e = lk;
+            int qx = 0, qy = 0;
+            nbr(e, [&](const auto& lx, const auto& ly, int ln) {
+                int op = point_index(hx[k] - 4 * edx(e), hy[k] - 4 * edy(e), lx, ly, ln);
+                q_point(lx, ly, ln, op, &qx, &qy);
+                return 0;
+            });
             qx += 64 * edx(e); qy += 64 * edy(e);
             int rx, ry;
             r_point(hx, hy, n, k - 1, &rx, &ry);
-            ox[out] = rx; oy[out] = ry; out++;
-            ox[out] = (rx + qx) / 2; oy[out] = (ry + qy) / 2; out++;
+            emit(rx, ry);
+            emit((rx + qx) / 2, (ry + qy) / 2);
         } else {
-            ox[out] = 16 * hx[k]; oy[out] = 16 * hy[k]; out++;
+            emit(16 * hx[k], 16 * hy[k]);
         }
     }
+}
+
+// subdivide_cell_emit into arrays: nbr(e, lx, ly) -> ln copies the
+// neighbour's hull into lx, ly; the new polygon goes to ox, oy (2n entries);
+// returns its size.
+template <class NbrHull, class TJunction>
+__host__ __device__ inline int subdivide_cell(const int* hx, const int* hy, int n, uint32_t node, NbrHull nbr,
+                                              TJunction tjunction, int* ox, int* oy)
+{
+    int lx[2 * kMaxPts], ly[2 * kMaxPts];
+    int out = 0;
+    auto nbr_f = [&](int e, auto f) {
+        int ln = nbr(e, lx, ly);
+        return f((const int*)lx, (const int*)ly, ln);
+    };
+    subdivide_cell_emit(hx, hy, n, node, nbr_f, tjunction, [&](int x, int y) {
+        ox[out] = x;
+        oy[out] = y;
+        out++;
+    });
     return out;
 }
 

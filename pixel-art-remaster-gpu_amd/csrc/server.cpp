@@ -457,10 +457,16 @@ extern "C" int dpx_server_open_stream(dpx_server* srv, int width, int height, in
     const uint64_t per_frame = align256((size_t)width * channels * height) +
                                align256((size_t)pixels * params->scale * params->scale * 4);
     const int items = (int)std::max<uint64_t>(1, std::min<uint64_t>(srv->max_batch, srv->budget / per_frame));
-    for (Lane& lane : srv->lanes)
+    for (Lane& lane : srv->lanes) {
         if (int rc = dpx_context_reserve(lane.ctx, items, std::max(pixels, srv->reserved_pixels.load()),
                                          params->scale))
             return rc;
+        // subdivided frames also need cell rows (a word per sample row and one more per cell)
+        if (params->subdivide)
+            if (int rc = context_reserve_rows(lane.ctx, (size_t)items * align256((size_t)pixels * 4 *
+                                                                                 cell_row_words(params->scale))))
+                return rc;
+    }
     std::lock_guard<std::mutex> lock(srv->mu);
     DPX_CHECK(!srv->closing, DPX_ECLOSED, "server closing");
     srv->reserved_pixels = std::max(pixels, srv->reserved_pixels.load());

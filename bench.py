@@ -226,6 +226,48 @@ def bench_served_streams(args, D, n_streams=128, hz=60.0, seconds=2.5):
     print(json.dumps(out), flush=True)
 
 
+def bench_components(args, torch, D):
+    """F1: component labels and border chains of one resolved graph per call
+    (dpx_stage_components, dpx_stage_border_chains), timed with events on the
+    launch stream; 256x224 and 640x480 frames."""
+    import ctypes
+    stream = torch.cuda.current_stream()
+    res = {}
+    for kind, seed, W, H in ((0, 1, 256, 224), (4, 2, 640, 480)):
+        img = D.synth_frame(kind, seed, W, H, 3)
+        _, g, _ = D.frame(img, W, H, 3, scale=1)
+        d_g = torch.from_numpy(np.ascontiguousarray(g).reshape(-1)).to("cuda:0")
+        lab = torch.empty(W * H, dtype=torch.int32, device="cuda:0")
+        scratch = torch.empty(W * H, dtype=torch.uint8, device="cuda:0")
+        nodes = torch.empty(8 * W * H, dtype=torch.int32, device="cuda:0")
+        sizes = torch.empty(W * H, dtype=torch.int32, device="cuda:0")
+        counts = torch.zeros(3, dtype=torch.int32, device="cuda:0")
+        calls = {
+            "components": lambda: D.check(D.lib.dpx_stage_components(d_g.data_ptr(), W, H, lab.data_ptr(),
+                                                                      stream.cuda_stream)),
+            "border_chains": lambda: D.check(D.lib.dpx_stage_border_chains(
+                d_g.data_ptr(), W, H, scratch.data_ptr(), nodes.data_ptr(), 8 * W * H, sizes.data_ptr(), W * H,
+                counts.data_ptr(), stream.cuda_stream)),
+        }
+        for name, fn in calls.items():
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            n = 20
+            a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            a.record(stream)
+            for _ in range(n):
+                fn()
+            b.record(stream)
+            torch.cuda.synchronize()
+            ms = a.elapsed_time(b) / n
+            res[f"{name}_{W}x{H}_ms"] = round(ms, 4)
+        res[f"chains_{W}x{H}"] = int(counts[0].item())
+        res[f"chain_nodes_{W}x{H}"] = int(counts[1].item())
+    print(json.dumps({"metric": "F1 stage times (components, border chains)", "unit": "ms per frame",
+                      "n_gpus": 1, "data": "synthetic", **res}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -240,7 +282,7 @@ def main():
                     help="also run a plain 1 GiB device copy (reference for HBM byte counters)")
     ap.add_argument("--subdivide", action="store_true",
                     help="corner-cutting subdivision on (the reference's 's' key): c0, c1, c2, c4")
-    ap.add_argument("--config", default="c0", choices=["c0", "c1", "c2", "c3", "c4"],
+    ap.add_argument("--config", default="c0", choices=["c0", "c1", "c2", "c3", "c4", "f1"],
                     help="c0 (default): the headline 256x224 -> 4x line; c1 4096 alpha sprites -> 6x; "
                          "c2 64 frames of five console sizes -> 4x; c3 128 served 320x224 streams at 60 Hz; "
                          "c4 16 640x480 frames -> 8x")
@@ -258,6 +300,8 @@ def main():
         return bench_batch_config(args, torch, D)
     if args.config == "c3":
         return bench_served_streams(args, D)
+    if args.config == "f1":
+        return bench_components(args, torch, D)
     dev = torch.device("cuda:0")
     stream = torch.cuda.current_stream()
 

@@ -222,6 +222,17 @@ int dpx_context_reserve(dpx_context* ctx, int max_items, int64_t max_item_pixels
  * a captured graph would still point at.  Capture dpx_plan_run instead. */
 int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_out, const dpx_item* items,
                   int n_items, void* stream);
+/* Raster tile shape per (size class, scale, subdivision): TX x (256 / TX)
+ * cells per block, TX = 16, 32 (the default) or 64; the reference's fixed
+ * 2x2-thread blocks (kernel.cu:392-394) have no counterpart here.  A context
+ * starts with the picks of the tuning file the autotuner writes
+ * (pixel-art-remaster-gpu_amd/tuning.json, or the file $DPX_TUNING names);
+ * these calls override them.  tile_w 0 restores the default. */
+#define DPX_CLASS_SPRITE 0  /* width and height <= 64                 */
+#define DPX_CLASS_FRAME  1  /* otherwise width and height <= 512      */
+#define DPX_CLASS_LARGE  2  /* larger                                  */
+int dpx_context_set_tile(dpx_context* ctx, int size_class, int scale, int subdivide, int tile_w);
+int dpx_context_get_tile(dpx_context* ctx, int size_class, int scale, int subdivide, int* tile_w);
 /* Whether the batches that follow also write the resolved graph of every
  * item (1 byte per pixel, the reference's graph_h): off by default, since
  * the images are the product and the graph writes cost about 3.5% of the

@@ -27,42 +27,21 @@
 // crossings of each tile, smooths and ear-clips its non-internal cells
 // (polygons in LDS, each own cell once) and writes their coverage rows; the
 // raster (kf_raster<S, true>) then reads the resolved graph and those rows.
+#include <dlfcn.h>
+#include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
 #include <map>
 #include <mutex>
+#include <string>
 #include <type_traits>
 #include <vector>
 
-#include "common.hpp"
-#include "geom.hpp"
-#include "graph.hpp"
-#include "pixel.hpp"
+#include "batch.hpp"
 
 namespace dpx {
-
-typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
-typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
-
-[[maybe_unused]] constexpr int kMaxScale = 15;
-[[maybe_unused]] constexpr int kKeys = 4096;
-constexpr int kHullBytes = 32;  // n, x[8], y[8] (quarter units), padding
-
-__host__ __device__ constexpr int mask_kmin(int S) { return -(S / 2) - 1; }
-__host__ __device__ constexpr int mask_w(int S) { return 2 * S + 2; }
-
-struct DevItem {
-    uint64_t in_off, out_off, g_off;
-    int32_t W, H, stride, ch;
-    uint32_t flags;
-    int32_t out_stride, scale, subdivide;
-    int32_t aligned16;  // output rows 16-byte aligned
-    int32_t graph_w4;   // graph rows 4-byte aligned (W % 4 == 0)
-    uint64_t pal_off;   // DPX_INDEXED: the palette's byte offset in the input arena
-    uint64_t rows_off;  // subdivision: byte offset of the item's cell rows (kf_smooth)
-};
 
 // ---------------------------------------------------------------------------
 // tables
@@ -118,34 +97,12 @@ __global__ void kt_mask(const int8_t* __restrict__ hull_table, int S, uint32_t* 
     mask[gid] = coverage_row(x, y, tri, nt, S, r);
 }
 
-// Summary of a cell's coverage rows at scale S: bit 0 = it covers every
-// sample of its own square; bit 1 + (dy+1)*3 + (dx+1) = it covers a sample of
-// the neighbouring square at (dx, dy).
-__host__ __device__ inline uint32_t coverage_info(const uint32_t* rows, int S)
-{
-    const int kmin = mask_kmin(S), mw = mask_w(S);
-    const int c0 = -kmin;  // bit of the own square's first column
-    const uint32_t own = ((1u << S) - 1u) << c0;
-    uint32_t info = 0;
-    bool full = true;
-    for (int r = 0; r < mw; r++) {
-        int ky = kmin + r;
-        int dy = ky < 0 ? -1 : (ky >= S ? 1 : 0);
-        uint32_t row = rows[r];
-        if (dy == 0 && (row & own) != own) full = false;
-        if (row & ((1u << c0) - 1u)) info |= 1u << (1 + (dy + 1) * 3 + 0);
-        if ((row & own) && dy != 0) info |= 1u << (1 + (dy + 1) * 3 + 1);
-        if (row >> (c0 + S)) info |= 1u << (1 + (dy + 1) * 3 + 2);
-    }
-    return info | (full ? 1u : 0u);
-}
 
 // Cell masks (scales up to kMaxCellScale): entry j = (1 - dy) * 3 + (1 - dx)
 // of key k holds, as bit ky * S + q, whether a cell of key k covers sample q
 // of row ky of the cell at (-dx, -dy) from it, i.e. what k contributes when
 // it is the candidate at (dx, dy) of that cell.  They follow the row table
 // in the same allocation.
-constexpr int kMaxCellScale = 2;
 __global__ void kt_cell(const uint32_t* __restrict__ mask, int S, uint16_t* __restrict__ cell)
 {
     int gid = blockIdx.x * blockDim.x + threadIdx.x;
@@ -171,54 +128,8 @@ __global__ void kt_info(const uint32_t* __restrict__ mask, int S, uint16_t* __re
     info[key] = (uint16_t)coverage_info(mask + (size_t)key * mask_w(S), S);
 }
 
-// A cell whose own square it covers fully, and into which none of the four
-// cells drawn after it (up-left, up, up-right, right) reaches, shows its own
-// colour on all of its S*S output pixels.
-__device__ inline bool simple_cell(uint32_t own, uint32_t upl, uint32_t up, uint32_t upr, uint32_t right)
-{
-    return (own & 1u) && !((upl >> 3) & 1u) && !((up >> 2) & 1u) && !((upr >> 1) & 1u) && !((right >> 4) & 1u);
-}
 
 // ---------------------------------------------------------------------------
-// work distribution: block -> (item, tile column, tile row).  When every item
-// of a launch has the same tile grid, the launch grid is (tile columns, tile
-// rows, items) and the block index says it all; otherwise the grid is 1-D
-// over all tiles and a map built on the host gives each block its item and
-// tile: tile_map[b] = (item, tile row << 16 | tile column).
-// ---------------------------------------------------------------------------
-struct TileAt {
-    int item, tx, ty;
-};
-
-// With RUN > 1 the blocks are permuted in groups of 8 * RUN: the hardware
-// hands consecutive blocks to the 8 XCDs in turn, and within a group XCD x
-// is given the RUN consecutive tiles x * RUN .. x * RUN + RUN - 1 (a row of
-// horizontally adjacent tiles), so the input and graph lines that adjacent
-// tiles share are fetched into one XCD's L2 once, not into several.  Blocks
-// past the last whole group keep their place.
-template <int RUN>
-__device__ inline TileAt locate_tile(const uint2* __restrict__ tile_map, int grid3d)
-{
-    if constexpr (RUN == 1) {
-        if (grid3d) return TileAt{ (int)blockIdx.z, (int)blockIdx.x, (int)blockIdx.y };
-        const uint2 m = tile_map[blockIdx.x];
-        return TileAt{ (int)m.x, (int)(m.y & 0xFFFFu), (int)(m.y >> 16) };
-    } else {
-        const uint32_t n = gridDim.x * gridDim.y * gridDim.z;
-        uint32_t b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-        if (b < n / (8 * RUN) * (8 * RUN)) {
-            const uint32_t w = b % (8 * RUN);
-            b += (w % 8) * RUN + w / 8 - w;
-        }
-        if (grid3d) {
-            const uint32_t plane = gridDim.x * gridDim.y, r = b % plane;
-            return TileAt{ (int)(b / plane), (int)(r % gridDim.x), (int)(r / gridDim.x) };
-        }
-        const uint2 m = tile_map[b];
-        return TileAt{ (int)m.x, (int)(m.y & 0xFFFFu), (int)(m.y >> 16) };
-    }
-}
-
 // ---------------------------------------------------------------------------
 // kf_graph: similarity graph + trivial crossing removal
 // ---------------------------------------------------------------------------
@@ -238,9 +149,6 @@ __device__ inline TileAt locate_tile(const uint2* __restrict__ tile_map, int gri
 //      from those bits of the neighbouring bytes.
 constexpr int GTX = 64, GTY = 32;
 constexpr int GLX = GTX + 3, GLY = GTY + 4;  // lane words
-// Lane-word rows in LDS: GLW words, column c + 3 holding x = x0 - 1 + c (so
-// that x0 - 4 .. x0 + 67, 18 aligned groups of 4 pixels, fit a row).
-constexpr int GLW = GTX + 8;
 constexpr int GHX = GTX + 2, GHY = GTY + 2;  // positions of pass 2
 constexpr int GHP = GTX + 8;                 // byte pitch of pass 2's rows: column c is x = x0 - 4 + c
 
@@ -248,7 +156,7 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
                                                 int grid3d, const uint8_t* __restrict__ in,
                                                 uint8_t* __restrict__ gaux)
 {
-    __shared__ __align__(16) uint32_t s_lane[GLW * GLY];
+    __shared__ uint32_t s_lane[GLX * GLY];
     __shared__ uint32_t s_h32[GHY * GHP / 4];
     uint8_t* s_h = reinterpret_cast<uint8_t*>(s_h32);
     const int t = threadIdx.x;
@@ -279,58 +187,12 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         if (alpha && base[off + 3] == 0) return kLaneTransparent;
         return yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque;
     };
-    // Rows of 3 or 4 channels that start on 4-byte boundaries are read as
-    // groups of four pixels (3 or 4 dwords per group): x0 - 4 .. x0 + 67,
-    // GLY rows, 18 groups a row; groups that straddle the frame's left or
-    // right edge are read pixel by pixel.
-    const bool grouped = !indexed && ((it.in_off | (uint64_t)it.stride) & 3) == 0;
-    if (grouped) {
-        for (int k = t; k < GLY * 18; k += 256) {
-            const int r = k / 18, g = k % 18;
-            const int x = x0 - 4 + 4 * g, y = y0 - 2 + r;
-            uint32_t lw[4] = { 0, 0, 0, 0 };
-            if ((unsigned)y < (unsigned)it.H && x < it.W && x + 4 > 0) {
-                const uint32_t rowoff = (uint32_t)(topdown ? it.H - 1 - y : y) * (uint32_t)it.stride;
-                if (x >= 0 && x + 3 < it.W) {
-                    const uint32_t* p = reinterpret_cast<const uint32_t*>(base + rowoff + (uint32_t)x * it.ch);
-                    uint32_t d[4];
-                    d[0] = p[0];
-                    d[1] = p[1];
-                    d[2] = p[2];
-                    d[3] = it.ch == 4 ? p[3] : 0u;
-                    auto byte = [&](int j) { return (d[j >> 2] >> (8 * (j & 3))) & 0xFFu; };
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        int c0, c1, c2;
-                        bool clear = false;
-                        if (it.ch == 3) {  // (constant byte indices; the order is a select)
-                            const int bA = (int)byte(3 * q), bC = (int)byte(3 * q + 2);
-                            c0 = rgb ? bC : bA;
-                            c1 = (int)byte(3 * q + 1);
-                            c2 = rgb ? bA : bC;
-                        } else {
-                            c0 = (int)((d[q] >> (8 * o0)) & 0xFFu);
-                            c1 = (int)((d[q] >> 8) & 0xFFu);
-                            c2 = (int)((d[q] >> (8 * o2)) & 0xFFu);
-                            clear = alpha && (d[q] >> 24) == 0;
-                        }
-                        lw[q] = clear ? kLaneTransparent : (yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque);
-                    }
-                } else {
-#pragma unroll
-                    for (int q = 0; q < 4; q++)
-                        if ((unsigned)(x + q) < (unsigned)it.W) lw[q] = lane_word(rowoff + (uint32_t)(x + q) * it.ch);
-                }
-            }
-            u32x4 v = { lw[0], lw[1], lw[2], lw[3] };
-            *reinterpret_cast<u32x4*>(&s_lane[r * GLW + 4 * g]) = v;
-        }
-    }
-    // Otherwise pixel by pixel: window column c (x = x0 - 1 + c), row r
-    // (y = y0 - 2 + r).  Columns 0..63: one per thread (t % 64), every fourth
-    // row from t / 64, the byte offset stepping by four rows; columns
-    // 64..GLX-1: one word per thread.
-    if (!grouped) {
+    // Window column c (x = x0 - 1 + c), row r (y = y0 - 2 + r).  Columns
+    // 0..63: one per thread (t % 64), every fourth row from t / 64, the byte
+    // offset stepping by four rows; columns 64..GLX-1: one word per thread.
+    // (Reading aligned rows as groups of four pixels, dwordx3 per thread and
+    // four conversions each, measured slower: 0.153 -> 0.160-0.173 ms.)
+    {
         const int c = t & 63, x = x0 - 1 + c;
         const bool xin = (unsigned)x < (unsigned)it.W;
         int y = y0 - 2 + (t >> 6);
@@ -340,22 +202,22 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         for (int r = t >> 6; r < GLY; r += 4, y += 4, off += step) {
             uint32_t lw = 0;
             if (xin && (unsigned)y < (unsigned)it.H) lw = lane_word(off);
-            s_lane[r * GLW + c + 3] = lw;
+            s_lane[r * GLX + c] = lw;
         }
     }
-    if (!grouped && t < (GLX - 64) * GLY) {
+    if (t < (GLX - 64) * GLY) {
         const int c = 64 + t % (GLX - 64), r = t / (GLX - 64);
         const int x = x0 - 1 + c, y = y0 - 2 + r;
         uint32_t lw = 0;
         if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
             lw = lane_word((uint32_t)((topdown ? it.H - 1 - y : y) * it.stride + x * it.ch));
-        s_lane[r * GLW + c + 3] = lw;
+        s_lane[r * GLX + c] = lw;
     }
     __syncthreads();
     // pass 2: position (x0 - 1 + cx, y0 - 1 + cy), same column split
     auto edges = [&](int cx, int cy) {
-        const int c = (cy + 1) * GLW + cx + 3;  // its lane word
-        uint32_t up = s_lane[c + GLW], ur = s_lane[c + GLW + 1], rt = s_lane[c + 1], dr = s_lane[c - GLW + 1];
+        const int c = (cy + 1) * GLX + cx;  // its lane word
+        uint32_t up = s_lane[c + GLX], ur = s_lane[c + GLX + 1], rt = s_lane[c + 1], dr = s_lane[c - GLX + 1];
         LaneRef a = lane_ref(s_lane[c]);
         uint32_t h = lane_similar(a, up) << 1 | lane_similar(a, ur) << 2 | lane_similar(a, rt) << 4 |
                      lane_similar(a, dr) << 7;
@@ -402,859 +264,6 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
     }
 }
 
-// ---------------------------------------------------------------------------
-// kf_raster: heuristic crossing resolution, cells (+ subdivision) and the
-// gather raster of a 32x8-cell tile
-// ---------------------------------------------------------------------------
-constexpr int RTX = 32, RTY = 8;
-constexpr int RCX = RTX + 2, RCY = RTY + 2;  // candidate cells, halo 1
-constexpr uint16_t kNoCell = 0xFFFF;
-
-// LDS windows of the raster tile hold graph bytes in rows of RWW 32-bit
-// words; word w of a row holds x = x0 - 8 + 4w .. x0 - 5 + 4w.  Words 1..10
-// (x0 - 4 .. x0 + 35) carry data, words 0 and 11 stay zero.  Candidate cells
-// (halo 1) need the resolved bytes of their left and right neighbours for
-// their keys: resolved rows at halo 1; with subdivision the hulls at halo 2
-// are needed as well: resolved rows at halo 2.  Resolving a byte reads the
-// snapshot rows above and below it.
-constexpr int RWW = 12, RWB = 4 * RWW;
-static_assert(RTX * RTY == 256, "one own cell per thread of a 256-thread block");
-// Resolved rows y0 - 1 .. y0 + RTY (candidate cells need their left and
-// right neighbours' bytes for their keys); resolving them reads one more
-// snapshot row on each side.
-constexpr int GY = RTY + 2, AY = GY + 2;
-
-// Per-cell coverage rows of smoothed cells (kf_smooth -> kf_raster<S, true>):
-// MW words of sample rows and one word of coverage summary per cell, cell
-// (x, y) of an item at word (y * W + x) * (MW + 1) of its range.
-__host__ __device__ constexpr int rows_words(int S) { return mask_w(S) + 1; }
-
-// The snapshot rows ay0 .. ay0 + AY - 1 of a tile's window into LDS.  With
-// W % 4 == 0 the graph rows are word aligned and a word is either wholly
-// inside the frame or wholly outside.
-__device__ inline void tile_snapshot_load(const DevItem& it, const uint8_t* __restrict__ aux, int x0, int ay0,
-                                          uint32_t* s_aux32, int t)
-{
-    if (it.graph_w4) {
-        if (t < AY * RWW) {
-            int r = t / RWW, w = t % RWW;
-            int x = x0 - 8 + 4 * w, y = ay0 + r;
-            uint32_t v = 0;
-            if (w >= 1 && w <= 10 && (unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
-                v = *reinterpret_cast<const uint32_t*>(aux + (uint32_t)(y * it.W + x));
-            s_aux32[t] = v;
-        }
-    } else {
-        uint8_t* s8 = reinterpret_cast<uint8_t*>(s_aux32);
-        for (int k = t; k < AY * RWB; k += 256) {
-            int r = k / RWB, c = k % RWB;
-            int x = x0 - 8 + c, y = ay0 + r;
-            bool ok = c >= 4 && c < 44 && (unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H;
-            s8[k] = ok ? aux[(uint32_t)(y * it.W + x)] : 0;
-        }
-    }
-}
-
-// Heuristic crossing resolution (ambiguous_cross_Kernel) of a tile's window,
-// after tile_snapshot_load and a barrier.  Only the pixels of crossed 2x2
-// blocks (both diagonals present) change, and in the symmetric snapshot the
-// block with lower-left pixel p is crossed iff p has bit 2 and its right
-// neighbour bit 0.  1: list the crossed blocks of the window, four origins
-// of a row per thread; 2: decide each listed block once
-// (processHeuristics2), the walks reading the snapshot in LDS where they can
-// and in HBM beyond the window; 3: remove the losing diagonals, four
-// positions per thread, into s_g32 (resolved rows y0 - 1 .. y0 + RTY), and
-// the tile's own resolved bytes into gfin when it is given.  s_p32 must
-// have been cleared and *s_nblk zeroed before the barrier.
-__device__ inline void tile_resolve(const DevItem& it, const uint8_t* __restrict__ aux, uint8_t* __restrict__ gfin,
-                                    int x0, int y0, int ay0, const uint32_t* s_aux32, uint32_t* s_p32,
-                                    uint16_t* s_blk, int* s_nblk, uint32_t* s_g32, int t)
-{
-    for (int k = t; k < (GY + 1) * 10; k += 256) {
-        const int r = k / 10, w = k % 10 + 1;  // word w of snapshot row r: origins of blocks on rows r, r + 1
-        const uint32_t* a = s_aux32 + r * RWW + w;
-        uint32_t D = a[0];
-        uint32_t cross = (D >> 2) & __builtin_amdgcn_alignbyte(a[1], D, 1) & 0x01010101u;
-        while (cross) {
-            int q = __builtin_ctz(cross) >> 3;
-            cross &= cross - 1;
-            s_blk[atomicAdd(s_nblk, 1)] = (uint16_t)(r * RWB + 4 * w + q);
-        }
-    }
-    __syncthreads();
-    {
-        // (the window is read by words, so that the compiler keeps the LDS
-        // and HBM reads apart instead of merging them into one generic load)
-        auto get = [&](int gx, int gy) -> uint32_t {
-            int wx = gx - (x0 - 4), wy = gy - ay0;
-            bool win = (unsigned)wx < 40u && (unsigned)wy < (unsigned)AY;
-            int c = win ? wy * RWB + wx + 4 : 0;
-            uint32_t v = (s_aux32[c >> 2] >> (8 * (c & 3))) & 0xFFu;
-            if (!win) v = aux[(uint32_t)(gy * it.W + gx)];
-            return v;
-        };
-        const int nblk = *s_nblk;
-        uint8_t* p8 = reinterpret_cast<uint8_t*>(s_p32);
-        for (int k = t; k < nblk; k += 256) {
-            int e = s_blk[k];
-            int r = e / RWB, c = e % RWB;
-            // 1: "/" loses (bits 2 and 5), 2: "\" loses (bits 0 and 7)
-            p8[e] = remove_slash(get, x0 - 8 + c, ay0 + r) ? 1 : 2;
-        }
-    }
-    __syncthreads();
-    for (int k = t; k < GY * 10; k += 256) {
-        const int r = k / 10, w = k % 10 + 1;  // word w of resolved row r (snapshot row r + 1)
-        const int y = ay0 + 1 + r, x = x0 - 8 + 4 * w;
-        const uint32_t* pa = s_p32 + (r + 1) * RWW + w;  // decisions of the blocks with origin on row y
-        const uint32_t* pb = pa - RWW;                   // ... on row y - 1
-        uint32_t A = pa[0], Al = __builtin_amdgcn_alignbyte(A, pa[-1], 3);
-        uint32_t B = pb[0], Bl = __builtin_amdgcn_alignbyte(B, pb[-1], 3);
-        uint32_t clear = ((A & 0x01010101u) << 2) |        // up-right block: "/"
-                         ((Al >> 1) & 0x01010101u) |        // up-left block: "\"
-                         ((Bl & 0x01010101u) << 5) |        // down-left block: "/"
-                         (((B >> 1) & 0x01010101u) << 7);  // down-right block: "\"
-        uint32_t res = s_aux32[(r + 1) * RWW + w] & ~clear;
-        s_g32[r * RWW + w] = res;
-        // the resolved graph of the tile's own cells, when the caller keeps it
-        if (gfin && y >= y0 && y < y0 + RTY && w >= 2 && w <= 9 && y < it.H && x < it.W) {
-            uint8_t* dst = gfin + it.g_off + (uint32_t)(y * it.W + x);
-            if (it.graph_w4) {
-                *reinterpret_cast<uint32_t*>(dst) = res;
-            } else {
-                int n = min(4, it.W - x);
-                for (int q = 0; q < n; q++) dst[q] = (uint8_t)(res >> (8 * q));
-            }
-        }
-    }
-}
-
-// The resolved rows y0 - 1 .. y0 + RTY of a tile's window, read from the
-// resolved graph written by kf_smooth (words laid out as in s_g32).
-__device__ inline void tile_resolved_load(const DevItem& it, const uint8_t* __restrict__ gfin, int x0, int y0,
-                                          uint32_t* s_g32, int t)
-{
-    const uint8_t* g = gfin + it.g_off;
-    if (it.graph_w4) {
-        if (t < GY * RWW) {
-            int r = t / RWW, w = t % RWW;
-            int x = x0 - 8 + 4 * w, y = y0 - 1 + r;
-            uint32_t v = 0;
-            if (w >= 1 && w <= 10 && (unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
-                v = *reinterpret_cast<const uint32_t*>(g + (uint32_t)(y * it.W + x));
-            s_g32[t] = v;
-        }
-    } else {
-        uint8_t* s8 = reinterpret_cast<uint8_t*>(s_g32);
-        for (int k = t; k < GY * RWB; k += 256) {
-            int r = k / RWB, c = k % RWB;
-            int x = x0 - 8 + c, y = y0 - 1 + r;
-            bool ok = c >= 4 && c < 44 && (unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H;
-            s8[k] = ok ? g[(uint32_t)(y * it.W + x)] : 0;
-        }
-    }
-}
-
-// ---------------------------------------------------------------------------
-// kf_smooth: crossing resolution, cells and corner-cutting subdivision of a
-// 32x8-cell tile (subdivision_Kernel kernel.cu:216-261 on the cells_Kernel
-// hulls, then triangulate_Kernel's ear clipping, kernel.cu:264-282), each
-// own cell once.  Writes the tile's resolved graph bytes to gfin and, for
-// every smoothed (non-internal) cell, the coverage rows of its triangles at
-// scale S with their summary to `rows` (rows_words(S) words per cell).
-// Internal cells (node 90) keep their hull (kernel.cu:231): the raster takes
-// their rows from the table.  The cells that need smoothing are listed first
-// so that every lane of a wave works on one; each lane's polygon lives in
-// LDS (a column of s_poly), where ear clipping scans and shifts it.
-// ---------------------------------------------------------------------------
-template <int S>
-__global__ __launch_bounds__(256) void kf_smooth(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
-                                                 int grid3d, const uint8_t* __restrict__ in,
-                                                 const uint8_t* __restrict__ gaux, uint8_t* __restrict__ gfin,
-                                                 const int8_t* __restrict__ hull_table, uint32_t* __restrict__ rows)
-{
-    constexpr int MW = mask_w(S), KMIN = mask_kmin(S), RW = rows_words(S);
-    constexpr int HB = 17;  // LDS bytes per hull: n, x[8], y[8] (quarter units)
-    __shared__ uint32_t s_aux32[AY * RWW];
-    __shared__ uint32_t s_p32[(GY + 1) * RWW];
-    __shared__ uint16_t s_blk[(GY + 1) * 40];
-    __shared__ int s_nblk;
-    __shared__ uint32_t s_g32[GY * RWW];
-    __shared__ int8_t s_hull[RCX * RCY * HB];
-    __shared__ uint16_t s_list[RTX * RTY];
-    __shared__ int s_nlist;
-    __shared__ uint32_t s_poly[2 * kMaxPts * 256];  // point k of lane t at [k * 256 + t]: x | y << 16 (1/64 units)
-    const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
-
-    const int t = threadIdx.x;
-    const TileAt at = locate_tile<8>(tile_map, grid3d);
-    const DevItem it = items[at.item];
-    const int x0 = at.tx * RTX, y0 = at.ty * RTY;
-    const uint8_t* aux = gaux + it.g_off;
-    const int ay0 = y0 - 2;
-    if (t == 0) {
-        s_nlist = 0;
-        s_nblk = 0;
-    }
-    if (t < (GY + 1) * RWW) s_p32[t] = 0;
-    tile_snapshot_load(it, aux, x0, ay0, s_aux32, t);
-    __syncthreads();
-    tile_resolve(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, t);
-    __syncthreads();
-    // hulls of the own cells and of their neighbours: cell (x0 - 1 + lx,
-    // y0 - 1 + ly) at resolved row ly, byte lx + 7
-    for (int k = t; k < RCX * RCY; k += 256) {
-        const int lx = k % RCX, ly = k / RCX;
-        const int c = ly * RWB + lx + 7;
-        const uint32_t key = cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
-        const int8_t* e = hull_table + (size_t)key * kHullBytes;
-#pragma unroll
-        for (int q = 0; q < HB; q++) s_hull[k * HB + q] = e[q];
-    }
-    // own cells to smooth: in the frame and not internal
-    {
-        const int cx = t % RTX, cy = t / RTX;
-        const uint32_t node = s_g[(cy + 1) * RWB + cx + 8];
-        if (x0 + cx < it.W && y0 + cy < it.H && node != 90) s_list[atomicAdd(&s_nlist, 1)] = (uint16_t)t;
-    }
-    __syncthreads();
-    if (t >= s_nlist) return;
-    const int own = s_list[t];
-    const int cx = own % RTX, cy = own / RTX;
-    const int x = x0 + cx, y = y0 + cy;
-    const uint32_t node = s_g[(cy + 1) * RWB + cx + 8];
-    const int hk = (cy + 1) * RCX + cx + 1;
-    const int8_t* h = s_hull + hk * HB;
-    const int n = h[0];
-    struct HullX {
-        const int8_t* p;
-        __device__ int operator[](int k) const { return p[1 + k]; }
-    };
-    struct HullY {
-        const int8_t* p;
-        __device__ int operator[](int k) const { return p[9 + k]; }
-    };
-    ImgView im{ in + it.in_off, it.W, it.H, it.stride, it.ch, it.flags,
-                (it.flags & DPX_INDEXED) ? in + it.pal_off : nullptr };
-    const TjView tv = tj_view(im);
-    auto nbr = [&](int e, auto f) {
-        const int8_t* q = s_hull + (hk + edy(e) * RCX + edx(e)) * HB;
-        return f(HullX{ q }, HullY{ q }, (int)q[0]);
-    };
-    auto tj = [&](int x4, int y4) -> bool { return tjunction(tv, x, y, x4, y4); };
-    uint32_t* P = s_poly + t;  // this lane's column
-    int m = 0;
-    subdivide_cell_emit(HullX{ h }, HullY{ h }, n, node, nbr, tj, [&](int px, int py) {
-        P[256 * m] = ((uint32_t)px & 0xFFFFu) | ((uint32_t)py << 16);
-        m++;
-    });
-    auto px_of = [](uint32_t p) { return (int)(int16_t)(p & 0xFFFFu); };
-    auto py_of = [](uint32_t p) { return (int)p >> 16; };
-    // Ear clipping (process, triangulate_functions.cu:108-217) as ear_clip
-    // does it: vertices in counter-clockwise order (reversed when the
-    // signed area is not positive), first ear tried at (m-1, 0, 1), an ear
-    // needs a positive cross and no other vertex inside or on it, a failure
-    // counter of 2 nv.  Each ear's coverage rows are accumulated as it is cut.
-    uint32_t cov[MW];
-#pragma unroll
-    for (int r = 0; r < MW; r++) cov[r] = 0;
-    if (m >= 3) {
-        int a2 = 0;
-        for (int p = m - 1, q = 0; q < m; p = q++) {
-            const uint32_t A = P[256 * p], B = P[256 * q];
-            a2 += px_of(A) * py_of(B) - px_of(B) * py_of(A);
-        }
-        if (a2 <= 0)
-            for (int lo = 0, hi = m - 1; lo < hi; lo++, hi--) {
-                const uint32_t tmp = P[256 * lo];
-                P[256 * lo] = P[256 * hi];
-                P[256 * hi] = tmp;
-            }
-        int nv = m, count = 2 * nv;
-        for (int v = nv - 1; nv > 2;) {
-            if (0 >= (count--)) break;  // the reference gives up; the triangles cut so far stand
-            int u = v;
-            if (nv <= u) u = 0;
-            v = u + 1;
-            if (nv <= v) v = 0;
-            int w = v + 1;
-            if (nv <= w) w = 0;
-            const uint32_t A = P[256 * u], B = P[256 * v], Cc = P[256 * w];
-            const int ax = px_of(A), ay = py_of(A), bx = px_of(B), by = py_of(B), cx_ = px_of(Cc), cy_ = py_of(Cc);
-            bool ear = ((bx - ax) * (cy_ - ay) - (by - ay) * (cx_ - ax)) > 0;
-            if (ear) {
-                for (int p = 0; p < nv; p++) {
-                    if (p == u || p == v || p == w) continue;
-                    const uint32_t Q = P[256 * p];
-                    if (inside_tri(ax, ay, bx, by, cx_, cy_, px_of(Q), py_of(Q))) {
-                        ear = false;
-                        break;
-                    }
-                }
-            }
-            if (ear) {
-                // coverage of triangle (A, B, C) in units of 1/(128 S) of a
-                // cell; it is counter-clockwise (positive cross)
-                const int tax = ax * 2 * S, tay = ay * 2 * S, tbx = bx * 2 * S, tby = by * 2 * S;
-                const int tcx = cx_ * 2 * S, tcy = cy_ * 2 * S;
-                const int ymin = min(tay, min(tby, tcy)), ymax = max(tay, max(tby, tcy));
-#pragma unroll
-                for (int r = 0; r < MW; r++) {
-                    const int sy = (2 * (KMIN + r) + 1) * 64;
-                    if (sy < ymin || sy > ymax) continue;
-                    cov[r] |= tri_row_bits(tax, tay, tbx, tby, tcx, tcy, sy, KMIN, MW);
-                }
-                for (int s2 = v, t2 = v + 1; t2 < nv; s2++, t2++) P[256 * s2] = P[256 * t2];
-                nv--;
-                count = 2 * nv;
-            }
-        }
-    }
-    uint32_t* dst = rows + it.rows_off / 4 + (size_t)((uint32_t)y * (uint32_t)it.W + (uint32_t)x) * RW;
-#pragma unroll
-    for (int r = 0; r < MW; r++) dst[r] = cov[r];
-    dst[MW] = coverage_info(cov, S);
-}
-
-// `items` are the items of one (scale, subdivision) group, in launch order.
-// With SUB the resolved graph comes from gfin and the coverage rows of the
-// smoothed cells from `cell_rows` (kf_smooth ran before); without, the kernel
-// resolves the crossings itself from the snapshot.
-template <int S, bool SUB>
-__global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
-                                                 int grid3d,
-                                                 const uint8_t* __restrict__ in, const uint8_t* __restrict__ gaux,
-                                                 uint8_t* __restrict__ gfin, const uint32_t* __restrict__ cell_rows,
-                                                 const uint32_t* __restrict__ mask_table,
-                                                 const uint16_t* __restrict__ info_table, uint8_t* __restrict__ out)
-{
-    constexpr int MW = mask_w(S), KMIN = mask_kmin(S), RW = rows_words(S);
-    __shared__ uint32_t s_aux32[SUB ? 1 : AY * RWW];
-    __shared__ uint32_t s_p32[SUB ? 1 : (GY + 1) * RWW];  // decisions of crossed blocks, by origin
-    __shared__ uint16_t s_blk[SUB ? 1 : (GY + 1) * 40];   // crossed blocks: origin byte index in the window
-    __shared__ int s_nblk;
-    __shared__ uint32_t s_g32[GY * RWW];
-    __shared__ uint16_t s_key[RCX * RCY];
-    __shared__ uint8_t s_info[RCX * RCY];  // coverage summary bits 0..4 (simple_cell)
-    __shared__ uint32_t s_col[RCX * RCY];
-    __shared__ uint32_t s_src[SUB ? RCX * RCY : 1];  // smoothed candidates: word offset of their rows
-    __shared__ uint16_t s_list[RTX * RTY];  // own cells that need the candidate search
-    __shared__ int s_nlist;
-    // resolved pixels of the first listed cells, so that every own
-    // cell's rows are stored together (a line is then written by one store
-    // instruction instead of being split between the direct and searched
-    // cells).  A pixel is kept as its colour; from S = 6 a tile whose listed
-    // cells would overflow that keeps the code of each pixel's winning
-    // candidate instead (1 byte: (1 - dy) * 3 + (1 - dx), 9 = clear colour),
-    // four times as many cells.  Measured: codes cost a decode per pixel, so
-    // tiles that fit keep colours; at S = 5 codes lose to the overflow stores.
-    constexpr bool CODES = S >= 6;
-    constexpr int PX_WORDS = 2048, PX_CELLS = PX_WORDS / (S * S), PX_CODE_CELLS = 4 * PX_WORDS / (S * S);
-    static_assert(PX_CELLS >= 1, "at least one buffered cell");
-    __shared__ uint32_t s_px[PX_WORDS];
-    uint8_t* s_px8 = reinterpret_cast<uint8_t*>(s_px);
-    __shared__ uint16_t s_idx[RTX * RTY];    // list index of a listed own cell
-    __shared__ uint8_t s_simple[RTX * RTY];  // own cell shows its own colour only
-    const uint8_t* s_g = reinterpret_cast<const uint8_t*>(s_g32);
-
-    const int t = threadIdx.x;
-    const TileAt at = locate_tile<8>(tile_map, grid3d);
-    const DevItem it = items[at.item];
-    const int x0 = at.tx * RTX, y0 = at.ty * RTY;
-    const uint8_t* aux = gaux + it.g_off;
-    const int ay0 = y0 - 2;  // first snapshot row
-
-    if (t == 0) {
-        s_nlist = 0;
-        s_nblk = 0;
-    }
-    // The snapshot bytes (or the resolved ones) and the candidates' colours
-    // are independent loads: issue them together.
-    if constexpr (SUB) {
-        tile_resolved_load(it, gfin, x0, y0, s_g32, t);
-    } else {
-        if (t < (GY + 1) * RWW) s_p32[t] = 0;
-        tile_snapshot_load(it, aux, x0, ay0, s_aux32, t);
-    }
-    {
-        const uint8_t* base = in + it.in_off;
-        const uint32_t o0 = (it.flags & DPX_RGB) ? 2 : 0, o2 = 2 - o0;
-        const bool topdown = it.flags & DPX_TOPDOWN, four = it.ch == 4;
-        const bool indexed = it.flags & DPX_INDEXED;
-        const uint32_t* pal = reinterpret_cast<const uint32_t*>(in + it.pal_off);
-        for (int k = t; k < RCX * RCY; k += 256) {
-            int x = x0 - 1 + k % RCX, y = y0 - 1 + k / RCX;
-            if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
-                int row = topdown ? it.H - 1 - y : y;
-                // row * stride < 2^31 (checked on the host); the stride itself may exceed 2^24
-                uint32_t off = (uint32_t)row * (uint32_t)it.stride + __umul24(x, it.ch);
-                if (indexed) {
-                    const uint32_t e = pal[base[off]];
-                    s_col[k] = ((e >> (8 * o2)) & 0xFFu) | (e & 0xFF00u) | ((e >> (8 * o0)) & 0xFFu) << 16 |
-                               (e & 0xFF000000u);
-                } else {
-                    uint32_t a = four ? base[off + 3] : 255u;
-                    s_col[k] = (uint32_t)base[off + o2] | (uint32_t)base[off + 1] << 8 |
-                               (uint32_t)base[off + o0] << 16 | a << 24;  // colorPos (kernel.cu:98-101)
-                }
-            }
-        }
-    }
-    __syncthreads();
-    if constexpr (!SUB) {
-        tile_resolve(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, t);
-        __syncthreads();
-    }
-    // candidate cells: key and coverage summary.  Candidate (lx, ly) is the
-    // cell at (x0 - 1 + lx, y0 - 1 + ly): resolved row ly, byte lx + 7.
-    for (int k = t; k < RCX * RCY; k += 256) {
-        int lx = k % RCX, ly = k / RCX;
-        int x = x0 - 1 + lx, y = y0 - 1 + ly;
-        uint16_t key = kNoCell;
-        uint8_t info = 0;
-        if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H) {
-            int c = ly * RWB + lx + 7;
-            key = (uint16_t)cell_key(s_g[c], s_g[c - 1], s_g[c + 1]);
-            if constexpr (SUB) {
-                // a smoothed cell's rows and summary come from kf_smooth;
-                // an internal one (node 90) keeps its hull: the table's
-                uint32_t src = 0xFFFFFFFFu;
-                if ((key & 0xFFu) != 90) {
-                    src = (uint32_t)(it.rows_off / 4) + ((uint32_t)y * (uint32_t)it.W + (uint32_t)x) * RW;
-                    info = (uint8_t)(cell_rows[src + MW] & 0x1Fu);
-                } else {
-                    info = (uint8_t)(info_table[key] & 0x1Fu);
-                }
-                s_src[k] = src;
-            } else {
-                info = (uint8_t)(info_table[key] & 0x1Fu);
-            }
-        }
-        s_key[k] = key;
-        s_info[k] = info;
-    }
-    __syncthreads();
-
-    // Output pixels are written once and not read back by the kernels:
-    // non-temporal stores keep them from displacing the graph and table lines
-    // the other tiles' loads need in the caches.
-    const int OH = it.H * S;
-    auto store_row = [&](int gx, int Y, const uint32_t* col) {
-        int row = out_topdown(it.flags) ? (OH - 1 - Y) : Y;
-        uint8_t* dst = out + it.out_off + (size_t)row * it.out_stride + (size_t)gx * S * 4;
-        if constexpr (S % 4 == 0) {
-            if (it.aligned16) {
-#pragma unroll
-                for (int q = 0; q < S; q += 4) {
-                    u32x4 v = { col[q], col[q + 1], col[q + 2], col[q + 3] };
-                    __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(dst + 4 * q));
-                }
-                return;
-            }
-        }
-        if constexpr (S % 2 == 0) {
-            if ((reinterpret_cast<uintptr_t>(dst) & 7) == 0) {
-#pragma unroll
-                for (int q = 0; q < S; q += 2) {
-                    u32x2 v = { col[q], col[q + 1] };
-                    __builtin_nontemporal_store(v, reinterpret_cast<u32x2*>(dst + 4 * q));
-                }
-                return;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < S; q++) __builtin_nontemporal_store(col[q], reinterpret_cast<uint32_t*>(dst + 4 * q));
-    };
-
-    // Own cells, one per thread: a cell whose S*S pixels all show its own
-    // colour is written directly, the others are listed for the candidate
-    // search, which resolves their pixels into LDS first.
-    const int cx = t % RTX, cy = t / RTX;
-    const int kown = (cy + 1) * RCX + cx + 1;
-    const bool mine = x0 + cx < it.W && y0 + cy < it.H;
-    const bool simple = mine && simple_cell(s_info[kown], s_info[kown + RCX - 1], s_info[kown + RCX],
-                                            s_info[kown + RCX + 1], s_info[kown + 1]);
-    if (mine && !simple) {
-        const int i = atomicAdd(&s_nlist, 1);
-        s_list[i] = (uint16_t)t;
-        s_idx[t] = (uint16_t)i;
-    }
-    s_simple[t] = simple ? 1 : 0;
-    __syncthreads();
-
-    // One unit = one output row (S pixels) of a listed cell, resolved
-    // against the 3x3 candidates in draw order (highest cell index first).
-    constexpr uint32_t FULL = (S >= 32) ? 0xFFFFFFFFu : ((1u << S) - 1u);
-    const int nunits = s_nlist * S;
-    // uniform over the block; at 6x and 7x only tiles far beyond the colour
-    // buffer gain from codes (measured: sprites lose 5% at a threshold of 1x)
-    const bool codes = CODES && s_nlist > (S >= 8 ? 1 : 3) * PX_CELLS;
-    const int px_cells = codes ? PX_CODE_CELLS : PX_CELLS;
-    auto fetch_rows = [&](int u, uint32_t* rows) {
-        int own = s_list[u / S], ky = u % S;
-        int ux = own % RTX, uy = own / RTX;
-        // an own cell that covers its whole square takes every pixel the
-        // cells drawn after it leave: the candidates drawn before it (left,
-        // down-left, down, down-right) can win none, and are not fetched
-        const bool own_full = s_info[(uy + 1) * RCX + ux + 1] & 1u;
-#pragma unroll
-        for (int dy = 1; dy >= -1; dy--) {
-#pragma unroll
-            for (int dx = 1; dx >= -1; dx--) {
-                int k = (uy + 1 + dy) * RCX + (ux + 1 + dx);
-                int r = ky - dy * S - KMIN;
-                uint32_t row = 0;
-                const bool before_own = dy < 0 || (dy == 0 && dx < 0);
-                if (!(own_full && before_own) && r >= 0 && r < MW && s_key[k] != kNoCell) {
-                    if constexpr (SUB) {
-                        const uint32_t src = s_src[k];
-                        row = src != 0xFFFFFFFFu ? cell_rows[src + r] : mask_table[(uint32_t)s_key[k] * MW + r];
-                    } else {
-                        row = mask_table[(uint32_t)s_key[k] * MW + r];
-                    }
-                }
-                rows[(1 - dy) * 3 + (1 - dx)] = row;
-            }
-        }
-    };
-    auto resolve_unit = [&](int u, const uint32_t* rows) {
-        int own = s_list[u / S], ky = u % S;
-        int ux = own % RTX, uy = own / RTX;
-        uint32_t col[S];
-        if constexpr (CODES) {
-            if (codes) {  // the same search, keeping candidate codes
-                uint8_t code[S];
-#pragma unroll
-                for (int q = 0; q < S; q++) code[q] = 9;
-                uint32_t assigned = 0;
-#pragma unroll
-                for (int dy = 1; dy >= -1; dy--) {
-#pragma unroll
-                    for (int dx = 1; dx >= -1; dx--) {
-                        uint32_t row = rows[(1 - dy) * 3 + (1 - dx)];
-                        int sh = -dx * S - KMIN;
-                        uint32_t nb = (sh >= 0 ? (row >> sh) : (row << -sh)) & FULL & ~assigned;
-                        if (nb) {
-#pragma unroll
-                            for (int q = 0; q < S; q++)
-                                if ((nb >> q) & 1u) code[q] = (uint8_t)((1 - dy) * 3 + (1 - dx));
-                            assigned |= nb;
-                        }
-                    }
-                }
-                if (u / S < PX_CODE_CELLS) {
-#pragma unroll
-                    for (int q = 0; q < S; q++) s_px8[u * S + q] = code[q];
-                } else {
-#pragma unroll
-                    for (int q = 0; q < S; q++)
-                        col[q] = code[q] == 9 ? 0xFF000000u
-                                              : s_col[(uy + 2 - code[q] / 3) * RCX + (ux + 2 - code[q] % 3)];
-                    store_row(x0 + ux, (y0 + uy) * S + ky, col);  // beyond the buffer: written directly
-                }
-                return;
-            }
-        }
-#pragma unroll
-        for (int q = 0; q < S; q++) col[q] = 0xFF000000u;
-        uint32_t assigned = 0;
-#pragma unroll
-        for (int dy = 1; dy >= -1; dy--) {
-#pragma unroll
-            for (int dx = 1; dx >= -1; dx--) {
-                uint32_t row = rows[(1 - dy) * 3 + (1 - dx)];
-                int sh = -dx * S - KMIN;
-                uint32_t bits = (sh >= 0 ? (row >> sh) : (row << -sh)) & FULL;
-                uint32_t nb = bits & ~assigned;
-                if (nb) {
-                    uint32_t c = s_col[(uy + 1 + dy) * RCX + (ux + 1 + dx)];
-#pragma unroll
-                    for (int q = 0; q < S; q++)
-                        if ((nb >> q) & 1u) col[q] = c;
-                    assigned |= nb;
-                }
-            }
-        }
-        if (u / S < PX_CELLS) {
-#pragma unroll
-            for (int q = 0; q < S; q++) s_px[u * S + q] = col[q];
-        } else {
-            store_row(x0 + ux, (y0 + uy) * S + ky, col);  // beyond the buffer: written directly
-        }
-    };
-
-    if constexpr (!SUB && S <= kMaxCellScale) {
-        // 1x and 2x: one thread per listed cell, the candidates' samples of
-        // it from the cell-mask table (nine independent loads), resolved in
-        // the same draw order as the rows of the other scales.  Measured: 2x
-        // frames 8-11% faster than with row units; at 4x a thread resolving
-        // 16 pixels alone lengthens sparse tiles (-3.5% on 16-colour frames).
-        const uint16_t* cell_table = reinterpret_cast<const uint16_t*>(mask_table + kKeys * MW);
-        const int nlist = s_nlist;
-        for (int i = t; i < nlist; i += 256) {
-            const int own = s_list[i];
-            const int ux = own % RTX, uy = own / RTX;
-            uint32_t m[9], c[9];
-#pragma unroll
-            for (int j = 0; j < 9; j++) {
-                const int k = (uy + 2 - j / 3) * RCX + (ux + 2 - j % 3);
-                const uint32_t key = s_key[k];
-                m[j] = key != kNoCell ? cell_table[key * 9 + j] : 0u;
-                c[j] = s_col[k];
-            }
-            uint32_t col[S * S];
-#pragma unroll
-            for (int p = 0; p < S * S; p++) col[p] = 0xFF000000u;
-            uint32_t assigned = 0;
-#pragma unroll
-            for (int j = 0; j < 9; j++) {
-                const uint32_t nb = m[j] & ~assigned;
-#pragma unroll
-                for (int p = 0; p < S * S; p++) col[p] = ((nb >> p) & 1u) ? c[j] : col[p];
-                assigned |= nb;
-            }
-            if (i < PX_CELLS) {
-#pragma unroll
-                for (int p = 0; p < S * S; p++) s_px[i * S * S + p] = col[p];
-            } else {
-#pragma unroll
-                for (int ky = 0; ky < S; ky++) store_row(x0 + ux, (y0 + uy) * S + ky, col + ky * S);
-            }
-        }
-        __syncthreads();
-    } else if (nunits > 0) {  // uniform: read from LDS after the barrier
-        for (int u = t; u < nunits; u += 256) {
-            uint32_t rows[9];
-            fetch_rows(u, rows);
-            resolve_unit(u, rows);
-        }
-        __syncthreads();
-    }
-
-    // At S = 1, 2 and 4 one output row of a cell is one store and the lanes
-    // of a wave store adjacent cells, so each thread stores its own cell.  At
-    // the other scales that would make every store instruction strided (S * 4
-    // bytes between lanes, each line written in pieces by several
-    // instructions), so the tile's output is stored in chunks of CW pixels
-    // inside one cell, chunk c by thread c % 256: adjacent lanes store
-    // adjacent chunks of a row.  Searched cells beyond the pixel buffer were
-    // stored by the search itself; their chunks are skipped.  Measured: at 5x
-    // to 7x, with chunks of fewer than 4 pixels, the chunk loop's own work
-    // outweighs the coalescing once a tile's cells overflow the colour buffer
-    // (dense frames, sprites), so such tiles keep the per-cell stores; at 8x
-    // and from 9x up (rows of 36 bytes and more) chunks win on every tile.
-    constexpr int CW = (S % 4 == 0) ? 4 : ((S % 2 == 0) ? 2 : 1);
-    // pixel (kx, ky) of listed cell i, own cell (lx, ly), from the buffer;
-    // use_codes (std::true_type: the tile keeps candidate codes) is fixed per
-    // tile, outside the store loops
-    auto buffered = [&](auto use_codes, int i, int kx, int ky, int lx, int ly) -> uint32_t {
-        if constexpr (decltype(use_codes)::value) {
-            const int code = s_px8[(i * S + ky) * S + kx];
-            return code == 9 ? 0xFF000000u : s_col[(ly + 2 - code / 3) * RCX + (lx + 2 - code % 3)];
-        } else {
-            return s_px[(i * S + ky) * S + kx];
-        }
-    };
-    bool chunked = false, wide = false;
-    if constexpr (S != 1 && S != 2 && S != 4) {
-        const uintptr_t base = reinterpret_cast<uintptr_t>(out + it.out_off);
-        chunked = ((base | (uintptr_t)it.out_stride) & (4 * CW - 1)) == 0 &&
-                  (CW == 4 || S >= 9 || s_nlist <= PX_CELLS);
-        // odd scales: chunks of 4 pixels across cell boundaries when the rows
-        // are 16-byte aligned and every listed cell is buffered (no pixel of
-        // the tile was stored by the search)
-        if constexpr (CW == 1) wide = ((base | (uintptr_t)it.out_stride) & 15) == 0 && s_nlist <= px_cells;
-    }
-    // Chunks of 4 pixels of a tile row (16-byte stores) that may span two
-    // cells (S >= 3): each pixel takes its colour from its own cell, simple
-    // or buffered.  Measured against one-pixel chunks: 3x 14% and 5x 24%
-    // faster (7x: 8% before the path for two simple cells); against the
-    // two-pixel chunks of 6x, 6-12% slower (the per-pixel lookups cost more
-    // than the wider stores save).
-    auto store_wide = [&](auto use_codes) {
-        constexpr int CPR = RTX * S / 4;   // chunks per output row of the tile
-        constexpr int NCH = RTY * S * CPR;
-        static_assert(RTX * S % 4 == 0, "whole chunks per tile row");
-        const int ncx = min(RTX, it.W - x0), rowpx = ncx * S;  // cells and pixels of a tile row in the frame
-        for (int c = t; c < NCH; c += 256) {
-            const int ry = c / CPR, px0 = 4 * (c % CPR);
-            const int ly = ry / S, ky = ry % S;
-            if (px0 >= rowpx || y0 + ly >= it.H) continue;
-            // the chunk's cells; past the frame's last column the pixels are
-            // not stored and take that column's cell
-            const int la = px0 / S, lb = min((px0 + 3) / S, ncx - 1);
-            const int oa = ly * RTX + la, ob = ly * RTX + lb;
-            const bool sa = s_simple[oa], sb = s_simple[ob];
-            const int ia = sa ? 0 : s_idx[oa], ib = sb ? 0 : s_idx[ob];  // simple: entry 0, ignored
-            const uint32_t va = s_col[(ly + 1) * RCX + la + 1], vb = s_col[(ly + 1) * RCX + lb + 1];
-            uint32_t col[4];
-            if (sa && sb) {  // both cells simple (most chunks): no buffer reads
-#pragma unroll
-                for (int q = 0; q < 4; q++) col[q] = px0 + q >= (la + 1) * S ? vb : va;
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    const int p = px0 + q;
-                    const bool inb = p >= (la + 1) * S;  // pixel in the second cell
-                    const int lx = inb ? lb : la, i = inb ? ib : ia;
-                    const int kx = min(p - lx * S, S - 1);  // (past the frame: not stored)
-                    const uint32_t bv = buffered(use_codes, i, kx, ky, lx, ly);
-                    col[q] = (inb ? sb : sa) ? (inb ? vb : va) : bv;
-                }
-            }
-            const int Y = (y0 + ly) * S + ky;
-            const int row = out_topdown(it.flags) ? (OH - 1 - Y) : Y;
-            uint8_t* dst = out + it.out_off + (size_t)row * it.out_stride + ((size_t)x0 * S + px0) * 4;
-            if (px0 + 3 < rowpx) {
-                u32x4 v = { col[0], col[1], col[2], col[3] };
-                __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(dst));
-            } else {
-#pragma unroll
-                for (int q = 0; q < 4; q++)
-                    if (px0 + q < rowpx) __builtin_nontemporal_store(col[q], reinterpret_cast<uint32_t*>(dst + 4 * q));
-            }
-        }
-    };
-    auto store_chunks = [&](auto use_codes) {
-        constexpr int CPC = S / CW;          // chunks per cell row
-        constexpr int CPR = RTX * CPC;       // chunks per output row of the tile
-        constexpr int NCH = RTY * S * CPR;   // chunks of the tile
-        for (int c = t; c < NCH; c += 256) {
-            const int ry = c / CPR, rc = c % CPR;
-            const int lx = rc / CPC, kx = (rc % CPC) * CW;
-            const int ly = ry / S, ky = ry % S;
-            if (x0 + lx >= it.W || y0 + ly >= it.H) continue;
-            const int own = ly * RTX + lx;
-            uint32_t col[CW];
-            if (s_simple[own]) {
-                const uint32_t v = s_col[(ly + 1) * RCX + lx + 1];
-#pragma unroll
-                for (int q = 0; q < CW; q++) col[q] = v;
-            } else {
-                const int i = s_idx[own];
-                if (i >= px_cells) continue;  // stored by the search
-#pragma unroll
-                for (int q = 0; q < CW; q++) col[q] = buffered(use_codes, i, kx + q, ky, lx, ly);
-            }
-            const int Y = (y0 + ly) * S + ky;
-            const int row = out_topdown(it.flags) ? (OH - 1 - Y) : Y;
-            uint8_t* dst = out + it.out_off + (size_t)row * it.out_stride + ((size_t)(x0 + lx) * S + kx) * 4;
-            if constexpr (CW == 4) {
-                u32x4 v = { col[0], col[1], col[2], col[3] };
-                __builtin_nontemporal_store(v, reinterpret_cast<u32x4*>(dst));
-            } else if constexpr (CW == 2) {
-                u32x2 v = { col[0], col[1] };
-                __builtin_nontemporal_store(v, reinterpret_cast<u32x2*>(dst));
-            } else {
-                __builtin_nontemporal_store(col[0], reinterpret_cast<uint32_t*>(dst));
-            }
-        }
-    };
-    // A thread's own cell, simple or buffered, through one sequence of store
-    // instructions for the whole wave: with separate branches for the two
-    // kinds, a wave holding both issues every row store twice, each writing
-    // part of the same lines.
-    auto store_cell = [&](auto use_codes, int i) {
-        const uint32_t c = s_col[kown];
-#pragma unroll
-        for (int ky = 0; ky < S; ky++) {
-            uint32_t col[S];
-#pragma unroll
-            for (int q = 0; q < S; q++) {
-                const uint32_t b = buffered(use_codes, i, q, ky, cx, cy);
-                col[q] = simple ? c : b;
-            }
-            store_row(x0 + cx, (y0 + cy) * S + ky, col);
-        }
-    };
-    if (wide) {
-        if constexpr (CODES) {
-            if (codes) {
-                store_wide(std::true_type{});
-                return;
-            }
-        }
-        store_wide(std::false_type{});
-        return;
-    }
-    if (chunked) {
-        if constexpr (CODES) {
-            if (codes) {
-                store_chunks(std::true_type{});
-                return;
-            }
-        }
-        store_chunks(std::false_type{});
-        return;
-    }
-    if (mine) {
-        const int i = simple ? 0 : s_idx[t];  // a simple cell reads entry 0 and ignores it
-        if (simple || i < px_cells) {         // listed cells beyond the buffer: stored by the search
-            if constexpr (CODES) {
-                if (codes) {
-                    store_cell(std::true_type{}, i);
-                    return;
-                }
-            }
-            store_cell(std::false_type{}, i);
-        }
-    }
-}
-
-using RasterFn = void (*)(const DevItem*, const uint2*, int, const uint8_t*, const uint8_t*, uint8_t*,
-                          const uint32_t*, const uint32_t*, const uint16_t*, uint8_t*);
-using SmoothFn = void (*)(const DevItem*, const uint2*, int, const uint8_t*, const uint8_t*, uint8_t*,
-                          const int8_t*, uint32_t*);
-
-// The raster kernel of scale S (1..15) with (SUB) or without subdivision.
-template <bool SUB>
-RasterFn pick_raster_of(int S)
-{
-    switch (S) {
-        case 1: return kf_raster<1, SUB>;
-        case 2: return kf_raster<2, SUB>;
-        case 3: return kf_raster<3, SUB>;
-        case 4: return kf_raster<4, SUB>;
-        case 5: return kf_raster<5, SUB>;
-        case 6: return kf_raster<6, SUB>;
-        case 7: return kf_raster<7, SUB>;
-        case 8: return kf_raster<8, SUB>;
-        case 9: return kf_raster<9, SUB>;
-        case 10: return kf_raster<10, SUB>;
-        case 11: return kf_raster<11, SUB>;
-        case 12: return kf_raster<12, SUB>;
-        case 13: return kf_raster<13, SUB>;
-        case 14: return kf_raster<14, SUB>;
-        case 15: return kf_raster<15, SUB>;
-    }
-    return nullptr;
-}
-
-RasterFn pick_raster(int S, bool sub) { return sub ? pick_raster_of<true>(S) : pick_raster_of<false>(S); }
-
-// The smoothing kernel of scale S (1..15).
-SmoothFn pick_smooth(int S)
-{
-    switch (S) {
-        case 1: return kf_smooth<1>;
-        case 2: return kf_smooth<2>;
-        case 3: return kf_smooth<3>;
-        case 4: return kf_smooth<4>;
-        case 5: return kf_smooth<5>;
-        case 6: return kf_smooth<6>;
-        case 7: return kf_smooth<7>;
-        case 8: return kf_smooth<8>;
-        case 9: return kf_smooth<9>;
-        case 10: return kf_smooth<10>;
-        case 11: return kf_smooth<11>;
-        case 12: return kf_smooth<12>;
-        case 13: return kf_smooth<13>;
-        case 14: return kf_smooth<14>;
-        case 15: return kf_smooth<15>;
-    }
-    return nullptr;
-}
-
 }  // namespace dpx
 
 
@@ -1268,6 +277,7 @@ SmoothFn pick_smooth(int S)
 struct PlanGroup {
     int scale;
     bool sub;
+    int tx;  // raster tile width (cells): 16, 32 or 64
     dim3 grid;
     int grid3d;
     size_t off_items, off_map;  // in the descriptor buffer
@@ -1302,6 +312,8 @@ struct dpx_context {
     std::vector<uint64_t> g_offsets;
     bool keep_graph = false;  // write the resolved graphs (dpx_context_keep_graph)
     bool last_kept = false;   // the last batch wrote them
+    // raster tile width per (size class, scale, subdivision); 0: the default
+    int8_t tile_pick[3][dpx::kMaxScale + 1][2] = {};
     std::mutex mu;
     // kernel timing: 3 events per recorded batch (start, after graph, end)
     bool timing = false;
@@ -1313,6 +325,76 @@ struct dpx_context {
 namespace {
 
 using namespace dpx;
+
+// Size class of an item for the tile picks.
+int size_class(int W, int H)
+{
+    if (W <= 64 && H <= 64) return DPX_CLASS_SPRITE;
+    if (W <= 512 && H <= 512) return DPX_CLASS_FRAME;
+    return DPX_CLASS_LARGE;
+}
+
+int item_tile(const dpx_context* ctx, const dpx_item& a)
+{
+    const int t = ctx->tile_pick[size_class(a.width, a.height)][a.scale][a.subdivide ? 1 : 0];
+    return t ? t : kDefaultTileX;
+}
+
+// The tile picks of a tuning file: a JSON object whose "picks" array holds
+// objects {"class": "sprite" | "frame" | "large", "scale": 1..15,
+// "subdivide": 0 | 1, "tile": 16 | 32 | 64} (other keys are ignored).
+// Returns the number of picks applied.
+int load_tuning(dpx_context* ctx, const char* path)
+{
+    FILE* f = fopen(path, "rb");
+    if (!f) return 0;
+    std::string text;
+    char buf[4096];
+    size_t n;
+    while ((n = fread(buf, 1, sizeof(buf), f)) > 0) text.append(buf, n);
+    fclose(f);
+    const size_t picks = text.find("\"picks\"");
+    if (picks == std::string::npos) return 0;
+    int applied = 0;
+    for (size_t a = text.find('{', picks); a != std::string::npos; a = text.find('{', a + 1)) {
+        const size_t b = text.find('}', a);
+        if (b == std::string::npos) break;
+        const std::string obj = text.substr(a, b - a);
+        auto num = [&](const char* key, int* v) {
+            const size_t k = obj.find(key);
+            if (k == std::string::npos) return false;
+            const size_t c = obj.find(':', k);
+            if (c == std::string::npos) return false;
+            *v = atoi(obj.c_str() + c + 1);
+            return true;
+        };
+        int cls = -1, scale = 0, sub = 0, tile = 0;
+        if (obj.find("\"sprite\"") != std::string::npos) cls = DPX_CLASS_SPRITE;
+        else if (obj.find("\"frame\"") != std::string::npos) cls = DPX_CLASS_FRAME;
+        else if (obj.find("\"large\"") != std::string::npos) cls = DPX_CLASS_LARGE;
+        if (cls < 0 || !num("\"scale\"", &scale) || !num("\"tile\"", &tile)) continue;
+        num("\"subdivide\"", &sub);
+        if (scale < 1 || scale > kMaxScale || (tile != 16 && tile != 32 && tile != 64)) continue;
+        ctx->tile_pick[cls][scale][sub ? 1 : 0] = (int8_t)tile;
+        applied++;
+    }
+    return applied;
+}
+
+// The tuning file: $DPX_TUNING, else tuning.json in the package directory
+// (one level above the directory of this library).
+std::string tuning_path()
+{
+    if (const char* e = getenv("DPX_TUNING")) return e;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<const void*>(&tuning_path), &info) && info.dli_fname) {
+        std::string p = info.dli_fname;
+        const size_t slash = p.rfind('/');
+        p = slash == std::string::npos ? std::string(".") : p.substr(0, slash);
+        return p + "/../tuning.json";
+    }
+    return "tuning.json";
+}
 
 int ensure_mask(dpx_context* ctx, int S)
 {
@@ -1334,6 +416,7 @@ int ensure_mask(dpx_context* ctx, int S)
 struct Group {
     int scale;
     bool sub;
+    int tx = kDefaultTileX;
     std::vector<DevItem> dev;  // the group's items, in launch order
     std::vector<uint2> map;    // tile map of a 1-D launch
     dim3 grid;
@@ -1410,8 +493,7 @@ size_t desc_bytes(size_t n)
 // tiles with a map from block to (item, tile row << 16 | tile column).
 // *grid3d tells the kernel which.  Tile columns and rows fit 16 bits: items
 // are at most 32768 pixels wide and high.
-template <int TX, int TY>
-dim3 tile_grid(const std::vector<DevItem>& dev, int* grid3d, std::vector<uint2>* map)
+dim3 tile_grid(const std::vector<DevItem>& dev, int TX, int TY, int* grid3d, std::vector<uint2>* map)
 {
     const unsigned tx0 = (unsigned)(dev[0].W + TX - 1) / TX, ty0 = (unsigned)(dev[0].H + TY - 1) / TY;
     bool same = dev.size() <= 65535;
@@ -1449,6 +531,7 @@ extern "C" int dpx_context_create(int device, dpx_context** out)
         return cleanup(fail(DPX_EHIP, "cell table build failed"));
     if (hipEventCreateWithFlags(&ctx->upload_done, hipEventDisableTiming) != hipSuccess)
         return cleanup(fail(DPX_EHIP, "event creation failed"));
+    load_tuning(ctx, tuning_path().c_str());
     *out = ctx;
     return DPX_OK;
 }
@@ -1492,6 +575,29 @@ int context_reserve_rows(dpx_context* ctx, size_t bytes)
     return grow_rows(ctx, bytes);
 }
 }  // namespace dpx
+
+extern "C" int dpx_context_set_tile(dpx_context* ctx, int size_class, int scale, int subdivide, int tile_w)
+{
+    DPX_CHECK(ctx, DPX_EINVAL, "null context");
+    DPX_CHECK(size_class >= 0 && size_class <= 2, DPX_EINVAL, "size class %d", size_class);
+    DPX_CHECK(scale >= 1 && scale <= kMaxScale, DPX_EINVAL, "scale %d outside 1..%d", scale, kMaxScale);
+    DPX_CHECK(tile_w == 0 || tile_w == 16 || tile_w == 32 || tile_w == 64, DPX_EINVAL, "tile width %d", tile_w);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->tile_pick[size_class][scale][subdivide ? 1 : 0] = (int8_t)tile_w;
+    ctx->plan.valid = false;  // the next batch plans its groups again
+    return DPX_OK;
+}
+
+extern "C" int dpx_context_get_tile(dpx_context* ctx, int size_class, int scale, int subdivide, int* tile_w)
+{
+    DPX_CHECK(ctx && tile_w, DPX_EINVAL, "bad arguments");
+    DPX_CHECK(size_class >= 0 && size_class <= 2, DPX_EINVAL, "size class %d", size_class);
+    DPX_CHECK(scale >= 1 && scale <= kMaxScale, DPX_EINVAL, "scale %d outside 1..%d", scale, kMaxScale);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    const int t = ctx->tile_pick[size_class][scale][subdivide ? 1 : 0];
+    *tile_w = t ? t : kDefaultTileX;
+    return DPX_OK;
+}
 
 extern "C" int dpx_context_keep_graph(dpx_context* ctx, int enable)
 {
@@ -1572,8 +678,10 @@ static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* it
             d.rows_off = roff;
             roff += align_up((size_t)a.width * a.height * rows_words(a.scale) * 4, 256);
         }
-        int gkey = a.scale * 2 + d.subdivide;
+        const int tx = item_tile(ctx, a);
+        const int gkey = tx << 8 | a.scale << 1 | d.subdivide;
         Group& grp = groups[gkey];
+        grp.tx = tx;
         grp.scale = a.scale;
         grp.sub = d.subdivide;
         grp.dev.push_back(d);
@@ -1584,8 +692,9 @@ static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* it
     // launch grids
     int g3d = 0;
     std::vector<uint2> gmap;
-    const dim3 ggrid = tile_grid<GTX, GTY>(dev, &g3d, &gmap);
-    for (auto& kv : groups) kv.second.grid = tile_grid<RTX, RTY>(kv.second.dev, &kv.second.grid3d, &kv.second.map);
+    const dim3 ggrid = tile_grid(dev, GTX, GTY, &g3d, &gmap);
+    for (auto& kv : groups)
+        kv.second.grid = tile_grid(kv.second.dev, kv.second.tx, 256 / kv.second.tx, &kv.second.grid3d, &kv.second.map);
 
     // descriptor blob: items | graph tile map | per group (items, tile map)
     blob.clear();
@@ -1604,7 +713,7 @@ static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* it
         const Group& grp = kv.second;
         size_t a = put(grp.dev.data(), grp.dev.size() * sizeof(DevItem));
         size_t b = put(grp.map.data(), grp.map.size() * sizeof(uint2));
-        plan.groups.push_back(PlanGroup{ grp.scale, grp.sub, grp.grid, grp.grid3d, a, b });
+        plan.groups.push_back(PlanGroup{ grp.scale, grp.sub, grp.tx, grp.grid, grp.grid3d, a, b });
     }
     plan.items.assign(items, items + n_items);
     plan.out_mod = reinterpret_cast<uintptr_t>(d_out) % 16;
@@ -1661,12 +770,12 @@ static int launch_plan(dpx_context* ctx, const BatchPlan& plan, const LaunchArgs
     hipLaunchKernelGGL(kf_graph, plan.ggrid, dim3(256), 0, stream, d_items, d_gmap, plan.g3d, d_in, la.gaux);
     if (ev_mid) DPX_HIP(hipEventRecord(ev_mid, stream));
     for (const PlanGroup& pg : plan.groups) {
-        RasterFn fn = pick_raster(pg.scale, pg.sub);
+        RasterFn fn = pick_raster(pg.scale, pg.sub, pg.tx);
         DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", pg.scale);
         const DevItem* d_gitems = reinterpret_cast<const DevItem*>(la.desc + pg.off_items);
         const uint2* d_rmap = reinterpret_cast<const uint2*>(la.desc + pg.off_map);
         if (pg.sub) {
-            SmoothFn sf = pick_smooth(pg.scale);
+            SmoothFn sf = pick_smooth(pg.scale, pg.tx);
             DPX_CHECK(sf && la.gfin && la.rows, DPX_EINVAL, "no smoothing buffers for scale %d", pg.scale);
             hipLaunchKernelGGL(sf, pg.grid, dim3(256), 0, stream, d_gitems, d_rmap, pg.grid3d, d_in, la.gaux, la.gfin,
                                ctx->d_hull, la.rows);
@@ -1797,13 +906,13 @@ hipKernelNodeParams node_params(dpx_plan* p, int kind, int group, const uint8_t*
             a.hull = p->ctx->d_hull;
             void* v[8] = { &a.items, &a.map, &a.g3d, &a.in, &a.gaux_c, &a.gfin, &a.hull, &a.rows };
             memcpy(a.ptrs, v, sizeof(v));
-            np.func = reinterpret_cast<void*>(pick_smooth(pg.scale));
+            np.func = reinterpret_cast<void*>(pick_smooth(pg.scale, pg.tx));
         } else {
             a.mask = p->ctx->d_mask[pg.scale];
             a.info = p->ctx->d_info[pg.scale];
             void* v[10] = { &a.items, &a.map, &a.g3d, &a.in, &a.gaux_c, &a.gfin, &a.rows_c, &a.mask, &a.info, &a.out };
             memcpy(a.ptrs, v, sizeof(v));
-            np.func = reinterpret_cast<void*>(pick_raster(pg.scale, pg.sub));
+            np.func = reinterpret_cast<void*>(pick_raster(pg.scale, pg.sub, pg.tx));
         }
         np.gridDim = pg.grid;
     }

@@ -457,4 +457,61 @@ __host__ __device__ inline uint32_t tri_row_bits(int ax, int ay, int bx, int by,
     return ((2u << hi) - 1u) & ~((1u << lo) - 1u);
 }
 
+// tri_row_bits of every row r = 0 .. MW - 1 at once (sample rows
+// sy = 128 (kmin + r) + 64), ORed into cov[r].  Along the rows an edge's
+// column bound moves by a fixed rational step, so each bound is kept as a
+// quotient and remainder and advanced by one exact step per row instead of
+// a division per row.  Same bits as tri_row_bits on every row.
+template <int MW>
+__host__ __device__ inline void tri_rows_or(int ax, int ay, int bx, int by, int cx, int cy, int kmin, uint32_t* cov)
+{
+    const int ymin = min(ay, min(by, cy)), ymax = max(ay, max(by, cy));
+    const int sy0 = 128 * kmin + 64;
+    // per edge: dy > 0 bounds hi, dy < 0 bounds lo, dy == 0 admits a range
+    // of rows.  q, rem: floor of the bound's numerator over D = 128 |dy|.
+    int q[3], rem[3], qs[3], rs[3], D[3], kind[3];  // kind: 1 hi, -1 lo, 0 horizontal
+    int ylo = sy0, yhi = sy0 + 128 * (MW - 1);       // rows the horizontal edges admit (sample y)
+    const int px_[3] = { ax, bx, cx }, py_[3] = { ay, by, cy };
+#pragma unroll
+    for (int e = 0; e < 3; e++) {
+        const int px = px_[e], py = py_[e], qx = px_[(e + 1) % 3], qy = py_[(e + 1) % 3];
+        const int dx = qx - px, dy = qy - py;
+        kind[e] = dy > 0 ? 1 : (dy < 0 ? -1 : 0);
+        D[e] = 128 * (dy < 0 ? -dy : dy);
+        q[e] = rem[e] = qs[e] = rs[e] = 0;
+        if (dy == 0) {
+            // K = dx (sy - py): covered iff K > 0, or K == 0 and dx > 0
+            if (dx > 0) ylo = max(ylo, py);          // sy >= py
+            else if (dx < 0) yhi = min(yhi, py - 1);  // sy < py
+            else yhi = ylo - 1;                      // (a point: nothing)
+            continue;
+        }
+        // numerator at row 0: N = dx (sy0 - py) + dy (px - sy0); hi uses N - 1, lo uses N
+        const int N0 = dx * (sy0 - py) + dy * (px - sy0) - (dy > 0 ? 1 : 0);
+        q[e] = floor_div(N0, D[e]);
+        rem[e] = N0 - q[e] * D[e];
+        qs[e] = floor_div(128 * dx, D[e]);
+        rs[e] = 128 * dx - qs[e] * D[e];
+    }
+#pragma unroll
+    for (int r = 0; r < MW; r++) {
+        const int sy = sy0 + 128 * r;
+        int lo = 0, hi = MW - 1;
+#pragma unroll
+        for (int e = 0; e < 3; e++) {
+            if (kind[e] > 0) hi = min(hi, q[e]);
+            else if (kind[e] < 0) lo = max(lo, -q[e]);
+            // advance to the next row
+            q[e] += qs[e];
+            rem[e] += rs[e];
+            if (rem[e] >= D[e] && kind[e] != 0) {
+                rem[e] -= D[e];
+                q[e]++;
+            }
+        }
+        if (sy >= ymin && sy <= ymax && sy >= ylo && sy <= yhi && lo <= hi)
+            cov[r] |= ((2u << hi) - 1u) & ~((1u << lo) - 1u);
+    }
+}
+
 }  // namespace dpx

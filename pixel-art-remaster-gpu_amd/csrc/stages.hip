@@ -424,27 +424,82 @@ __device__ inline int uf_find(int32_t* parent_, int v)
     return v;
 }
 
-__global__ void k_cc_init(int32_t* parent, int N)
+// Components in two levels: each 32x32 tile of nodes is labelled by one
+// block with a union-find in LDS over the edges inside the tile (roots are
+// the smallest node of each piece, so the tile's pieces come out with their
+// smallest global index); the edges that cross tile borders are then united
+// in global memory (k_cc_cross), and the final pass takes every node's root.
+constexpr int kCcTile = 32;
+
+__device__ inline int uf_find_lds(int* parent_, int v)
 {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k < N) parent[k] = k;
+    volatile int* parent = parent_;
+    int p = parent[v];
+    while (p != v) {
+        int gp = parent[p];
+        if (gp != p) parent[v] = gp;  // path halving; any ancestor is a valid parent
+        v = p;
+        p = gp;
+    }
+    return v;
 }
 
-__global__ void k_cc_union(const uint8_t* __restrict__ graph, int W, int H, int32_t* parent)
+__global__ __launch_bounds__(256) void k_cc_tile(const uint8_t* __restrict__ graph, int W, int H, int32_t* parent)
 {
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    int j = blockIdx.y * blockDim.y + threadIdx.y;
+    __shared__ int s_par[kCcTile * kCcTile];
+    const int tx0 = blockIdx.x * kCcTile, ty0 = blockIdx.y * kCcTile;
+    for (int k = threadIdx.x; k < kCcTile * kCcTile; k += 256) s_par[k] = k;
+    __syncthreads();
+    for (int k = threadIdx.x; k < kCcTile * kCcTile; k += 256) {
+        const int lx = k % kCcTile, ly = k / kCcTile, i = tx0 + lx, j = ty0 + ly;
+        if (i >= W || j >= H) continue;
+        const uint32_t g = graph[(size_t)j * W + i];
+        // edges towards smaller indices only (each edge once): bits 3, 5, 6, 7
+        const int es[4] = { 3, 5, 6, 7 };
+        for (int t = 0; t < 4; t++) {
+            const int e = es[t];
+            if (!bit(g, e)) continue;
+            const int nlx = lx + edx(e), nly = ly + edy(e);
+            if ((unsigned)nlx >= (unsigned)kCcTile || (unsigned)nly >= (unsigned)kCcTile) continue;  // k_cc_cross
+            if (!in_frame(i + edx(e), j + edy(e), W, H)) continue;
+            const int u = nly * kCcTile + nlx;
+            for (;;) {
+                const int ru = uf_find_lds(s_par, u), rv = uf_find_lds(s_par, k);
+                if (ru == rv) break;
+                const int hi = ru > rv ? ru : rv, lo = ru > rv ? rv : ru;
+                if (atomicCAS(&s_par[hi], hi, lo) == hi) break;
+            }
+        }
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < kCcTile * kCcTile; k += 256) {
+        const int lx = k % kCcTile, ly = k / kCcTile, i = tx0 + lx, j = ty0 + ly;
+        if (i >= W || j >= H) continue;
+        int r = k;
+        for (int p = s_par[r]; p != r; p = s_par[r]) r = p;
+        parent[(size_t)j * W + i] = (ty0 + r / kCcTile) * W + tx0 + r % kCcTile;
+    }
+}
+
+// The edges between tiles, united in global memory (parents are tile roots
+// after k_cc_tile).
+__global__ void k_cc_cross(const uint8_t* __restrict__ graph, int W, int H, int32_t* parent)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = blockIdx.y * blockDim.y + threadIdx.y;
     if (i >= W || j >= H) return;
-    int v = j * W + i;
-    uint32_t g = graph[v];
-    // edges towards smaller indices only (each edge once): bits 3, 5, 6, 7
+    const int lx = i % kCcTile, ly = j % kCcTile;
+    if (lx != 0 && ly != 0 && lx != kCcTile - 1) return;  // no backward edge leaves the tile
+    const int v = j * W + i;
+    const uint32_t g = graph[v];
     const int es[4] = { 3, 5, 6, 7 };
     for (int t = 0; t < 4; t++) {
-        int e = es[t];
+        const int e = es[t];
         if (!bit(g, e)) continue;
-        int ni = i + edx(e), nj = j + edy(e);
+        const int ni = i + edx(e), nj = j + edy(e);
         if (!in_frame(ni, nj, W, H)) continue;
-        int u = nj * W + ni;
+        if (ni / kCcTile == i / kCcTile && nj / kCcTile == j / kCcTile) continue;  // inside: k_cc_tile
+        const int u = nj * W + ni;
         for (;;) {
             int ru = uf_find(parent, u), rv = uf_find(parent, v);
             if (ru == rv) break;
@@ -493,12 +548,27 @@ __global__ void k_cc_final(int32_t* parent, int32_t* labels, int N)
 // (0, or DPX_ENOMEM when the output arrays are too small, or DPX_EINVAL when
 // a walk leaves the frame or does not close: the graph is not a resolved,
 // symmetric similarity graph).
-__global__ __launch_bounds__(64) void k_border_chains(const uint8_t* __restrict__ graph, int W, int H,
-                                                      uint8_t* mark, int32_t* nodes, int32_t* sizes, int max_nodes,
+//
+// The walk is a chain of dependent reads (each step's node decides the
+// next), so its speed is the latency of those reads: with LDS the frame's
+// graph bytes and marks are copied into the block's LDS first (2 bytes per
+// node, frames that fit), otherwise they are read from HBM (L2).
+template <bool LDS>
+__global__ __launch_bounds__(64) void k_border_chains(const uint8_t* __restrict__ graph_g, int W, int H,
+                                                      uint8_t* mark_g, int32_t* nodes, int32_t* sizes, int max_nodes,
                                                       int max_chains, int32_t* counts)
 {
+    extern __shared__ uint8_t s_dyn[];
     __shared__ int s_status;
     const int N = W * H, lane = threadIdx.x;
+    const uint8_t* graph = graph_g;
+    uint8_t* mark = mark_g;
+    if constexpr (LDS) {
+        uint8_t* g = s_dyn;
+        for (int k = lane; k < N; k += 64) g[k] = graph_g[k];
+        graph = g;
+        mark = s_dyn + N;
+    }
     for (int k = lane; k < N; k += 64) mark[k] = 0;
     if (lane == 0) s_status = 0;
     __syncthreads();
@@ -705,8 +775,9 @@ extern "C" int dpx_stage_components(const uint8_t* d_graph, int W, int H, int32_
     int N = W * H;
     hipStream_t s = as_stream(stream);
     // d_labels doubles as the parent array, then receives the roots
-    hipLaunchKernelGGL(k_cc_init, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, d_labels, N);
-    hipLaunchKernelGGL(k_cc_union, grid2(W, H), dim3(kB, kB), 0, s, d_graph, W, H, d_labels);
+    const dim3 tiles((unsigned)((W + kCcTile - 1) / kCcTile), (unsigned)((H + kCcTile - 1) / kCcTile));
+    hipLaunchKernelGGL(k_cc_tile, tiles, dim3(256), 0, s, d_graph, W, H, d_labels);
+    hipLaunchKernelGGL(k_cc_cross, grid2(W, H), dim3(kB, kB), 0, s, d_graph, W, H, d_labels);
     hipLaunchKernelGGL(k_cc_final, dim3((unsigned)((N + 255) / 256)), dim3(256), 0, s, d_labels, d_labels, N);
     DPX_HIP(hipGetLastError());
     return DPX_OK;
@@ -720,8 +791,21 @@ extern "C" int dpx_stage_border_chains(const uint8_t* d_graph, int W, int H, uin
               "bad arguments");
     DPX_CHECK((int64_t)W * H <= (1 << 28), DPX_EINVAL, "frame too large (%dx%d)", W, H);
     DPX_CHECK(max_nodes >= 0 && max_chains >= 0, DPX_EINVAL, "negative capacity");
-    hipLaunchKernelGGL(k_border_chains, dim3(1), dim3(64), 0, as_stream(stream), d_graph, W, H, d_scratch, d_nodes,
-                       d_sizes, max_nodes, max_chains, d_counts);
+    // frames whose graph and marks fit the block's LDS walk there
+    int dev = 0, lds_max = 0;
+    DPX_HIP(hipGetDevice(&dev));
+    DPX_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+    const size_t lds = 2 * (size_t)W * H;
+    if (lds + 1024 <= (size_t)lds_max &&
+        hipFuncSetAttribute(reinterpret_cast<const void*>(k_border_chains<true>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess) {
+        hipLaunchKernelGGL(k_border_chains<true>, dim3(1), dim3(64), lds, as_stream(stream), d_graph, W, H, d_scratch,
+                           d_nodes, d_sizes, max_nodes, max_chains, d_counts);
+    } else {
+        (void)hipGetLastError();
+        hipLaunchKernelGGL(k_border_chains<false>, dim3(1), dim3(64), 0, as_stream(stream), d_graph, W, H, d_scratch,
+                           d_nodes, d_sizes, max_nodes, max_chains, d_counts);
+    }
     DPX_HIP(hipGetLastError());
     return DPX_OK;
 }

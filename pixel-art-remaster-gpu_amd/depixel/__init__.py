@@ -19,6 +19,8 @@ HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "
 
 CELL_SIZE = 45
 RGB, TOPDOWN, ALPHA, FLIP_OUT, INDEXED = 1, 2, 4, 8, 16
+CLASS_SPRITE, CLASS_FRAME, CLASS_LARGE = 0, 1, 2
+TILE_WIDTHS = (16, 32, 64)
 VIEW_CELLS, VIEW_QUADS = 0, 1
 VIEW_AA, VIEW_TOPDOWN = 1, 2
 
@@ -98,6 +100,8 @@ _SIGS = {
     "dpx_batch_run": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(Item), c_int, c_void_p]),
     "dpx_batch_graph": (c_int, [c_void_p, POINTER(c_void_p), POINTER(POINTER(c_uint64))]),
     "dpx_context_keep_graph": (c_int, [c_void_p, c_int]),
+    "dpx_context_set_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "dpx_context_get_tile": (c_int, [c_void_p, c_int, c_int, c_int, POINTER(c_int)]),
     "dpx_context_reserve": (c_int, [c_void_p, c_int, ctypes.c_int64, c_int]),
     "dpx_batch_timing": (c_int, [c_void_p, c_int]),
     "dpx_plan_create": (c_int, [c_void_p, POINTER(Item), c_int, c_uint64, c_uint32, POINTER(c_void_p)]),
@@ -274,6 +278,15 @@ class Context:
     def reserve(self, max_items: int, max_item_pixels: int, scale: int = 0):
         """Sizes the context's buffers (and builds the tables of `scale`) ahead of the batches."""
         check(lib.dpx_context_reserve(self.h, max_items, max_item_pixels, scale))
+
+    def set_tile(self, size_class: int, scale: int, subdivide: bool, tile_w: int):
+        """Raster tile width (16, 32, 64; 0 = default) for items of a size class at a scale."""
+        check(lib.dpx_context_set_tile(self.h, size_class, scale, int(subdivide), tile_w))
+
+    def get_tile(self, size_class: int, scale: int, subdivide: bool) -> int:
+        t = c_int()
+        check(lib.dpx_context_get_tile(self.h, size_class, scale, int(subdivide), ctypes.byref(t)))
+        return t.value
 
     def timing(self, enable: bool):
         check(lib.dpx_batch_timing(self.h, int(enable)))

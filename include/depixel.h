@@ -253,6 +253,23 @@ typedef struct {
 int dpx_batch_timing(dpx_context* ctx, int enable);
 int dpx_batch_timing_read(dpx_context* ctx, dpx_batch_times* out);
 
+/* dpx_batch_run without any allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Every buffer beyond d_in and d_out (tables,
+ * descriptors, tile maps, the graph snapshot) lives in one static device
+ * array of the library, 58 MiB on each GPU it runs on, held until the
+ * process ends.  Limits: at most 4096 items, at most 48 MiB of graph bytes
+ * (one per pixel, each item rounded up to 256: e.g. 836 frames of 256x224),
+ * at most 65,536 graph tiles (64x32 pixels) and 524,288 raster tiles (32x8);
+ * larger batches are refused with DPX_EINVAL, and so are items with
+ * subdivision (it needs per-cell row buffers).  The resolved graph is not
+ * written.  Runs on the current device.  The first call on each device builds
+ * the tables and waits for them on `stream`, so it must not be captured.  All
+ * calls on one GPU share the array: calls that overlap on one GPU (on
+ * different streams, unsynchronised) race. */
+int dpx_batch_run_noalloc(const uint8_t* d_in, uint8_t* d_out, const dpx_item* items, int n_items,
+                          void* stream);
+
 /* Plans: a batch layout validated once, with descriptors and graph scratch
  * of its own (no counterpart in the reference, which renders one frame per
  * call).  dpx_plan_run launches the batch's kernels for any input and output

@@ -1065,3 +1065,208 @@ extern "C" int dpx_batch_timing_read(dpx_context* ctx, dpx_batch_times* out)
     ctx->acc = dpx_batch_times{};
     return DPX_OK;
 }
+
+// ---------------------------------------------------------------------------
+// dpx_batch_run_noalloc: the batched path with every buffer beyond the
+// caller's input and output in one static device array, `scratch`, so that a
+// call allocates nothing.  Layout (all offsets 256-byte aligned):
+//   tables     the hull table, and per scale 1..15 the coverage rows (with
+//              the cell masks at 1x and 2x) and the coverage summaries,
+//              built on the first call on each device;
+//   items      the batch's device items, in item order and grouped by
+//              (scale, subdivision);
+//   tile maps  block -> (item, tile) of the graph launch and of the raster
+//              launches;
+//   graph      the post-trivial snapshot graph, 1 byte per pixel.
+// A call's descriptors reach the device as kernel arguments (k_na_put,
+// kNaPutChunk items per launch), not as a copy from host memory.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr size_t na_align(size_t v) { return (v + 255) / 256 * 256; }
+constexpr size_t na_mask_bytes(int S)
+{
+    return na_align((size_t)kKeys * mask_w(S) * 4 + (S <= kMaxCellScale ? (size_t)kKeys * 9 * 2 : 0));
+}
+constexpr size_t na_info_bytes() { return na_align((size_t)kKeys * 2); }
+constexpr size_t na_mask_off(int S)
+{
+    size_t o = na_align((size_t)kKeys * kHullBytes);
+    for (int s = 1; s < S; s++) o += na_mask_bytes(s) + na_info_bytes();
+    return o;
+}
+constexpr size_t na_info_off(int S) { return na_mask_off(S) + na_mask_bytes(S); }
+
+constexpr int kNaMaxItems = 4096;
+constexpr int kNaTX = kDefaultTileX, kNaTY = 256 / kDefaultTileX;  // raster tiles (the default shape)
+constexpr size_t kNaGraphTiles = 65536, kNaRasterTiles = 524288;
+constexpr size_t kNaGraphBytes = 48u << 20;
+constexpr size_t kNaOffItems = na_mask_off(kMaxScale + 1);
+constexpr size_t kNaOffGroupItems = kNaOffItems + na_align(kNaMaxItems * sizeof(DevItem));
+constexpr size_t kNaOffGmap = kNaOffGroupItems + na_align(kNaMaxItems * sizeof(DevItem));
+constexpr size_t kNaOffRmap = kNaOffGmap + na_align(kNaGraphTiles * sizeof(uint2));
+constexpr size_t kNaOffGraph = kNaOffRmap + na_align(kNaRasterTiles * sizeof(uint2));
+constexpr size_t kNaScratchBytes = kNaOffGraph + kNaGraphBytes;
+static_assert(kNaScratchBytes < (64u << 20), "scratch stays under 64 MiB");
+
+static __device__ __align__(256) uint8_t scratch[kNaScratchBytes];
+
+// One item of a k_na_put launch: its descriptor, its slots in the item
+// arrays and the first entries of its tiles in the two tile maps.
+struct NaPut {
+    DevItem d;
+    uint32_t index;  // in the item array (the graph launch's items)
+    uint32_t gslot;  // in the grouped item array
+    uint32_t gpos;   // gslot - the group's first slot (the raster launch's items)
+    uint32_t gtile, rtile;
+};
+constexpr int kNaPutChunk = 32;
+struct NaPutArgs {
+    uint32_t n;
+    NaPut it[kNaPutChunk];
+};
+static_assert(sizeof(NaPutArgs) <= 4000, "kernel arguments stay under 4 KB");
+
+__global__ __launch_bounds__(256) void k_na_put(NaPutArgs a, DevItem* __restrict__ items,
+                                                DevItem* __restrict__ group_items, uint2* __restrict__ gmap,
+                                                uint2* __restrict__ rmap)
+{
+    for (uint32_t i = 0; i < a.n; i++) {
+        const NaPut& p = a.it[i];
+        if (threadIdx.x == 0) {
+            items[p.index] = p.d;
+            group_items[p.gslot] = p.d;
+        }
+        const uint32_t gx = (uint32_t)(p.d.W + GTX - 1) / GTX, gy = (uint32_t)(p.d.H + GTY - 1) / GTY;
+        for (uint32_t t = threadIdx.x; t < gx * gy; t += blockDim.x)
+            gmap[p.gtile + t] = make_uint2(p.index, (t / gx) << 16 | (t % gx));
+        const uint32_t rx = (uint32_t)(p.d.W + kNaTX - 1) / kNaTX, ry = (uint32_t)(p.d.H + kNaTY - 1) / kNaTY;
+        for (uint32_t t = threadIdx.x; t < rx * ry; t += blockDim.x)
+            rmap[p.rtile + t] = make_uint2(p.gpos, (t / rx) << 16 | (t % rx));
+    }
+}
+
+constexpr int kNaMaxDevices = 64;
+std::mutex g_na_mu;
+uint8_t* g_na_ptr[kNaMaxDevices] = {};  // each device's address of `scratch`
+
+// Set-up on the first call on a device: the array's address on this device
+// and its tables, built on `stream` and waited for (later calls on any
+// stream read them).
+int na_setup(int dev, hipStream_t stream, uint8_t** out)
+{
+    std::lock_guard<std::mutex> lock(g_na_mu);
+    if (!g_na_ptr[dev]) {
+        uint8_t* ptr = nullptr;
+        DPX_HIP(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)));
+        DPX_CHECK(ptr, DPX_EHIP, "no device address for the scratch array");
+        int8_t* hull = reinterpret_cast<int8_t*>(ptr);
+        hipLaunchKernelGGL(kt_hull, dim3(kKeys / 256), dim3(256), 0, stream, hull);
+        for (int S = 1; S <= kMaxScale; S++) {
+            uint32_t* mask = reinterpret_cast<uint32_t*>(ptr + na_mask_off(S));
+            const size_t n = (size_t)kKeys * mask_w(S);
+            hipLaunchKernelGGL(kt_mask, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, hull, S, mask);
+            if (S <= kMaxCellScale)
+                hipLaunchKernelGGL(kt_cell, dim3((unsigned)((kKeys * 9 + 255) / 256)), dim3(256), 0, stream, mask, S,
+                                   reinterpret_cast<uint16_t*>(mask + n));
+            hipLaunchKernelGGL(kt_info, dim3(kKeys / 256), dim3(256), 0, stream, mask, S,
+                               reinterpret_cast<uint16_t*>(ptr + na_info_off(S)));
+        }
+        DPX_HIP(hipGetLastError());
+        DPX_HIP(hipStreamSynchronize(stream));
+        g_na_ptr[dev] = ptr;
+    }
+    *out = g_na_ptr[dev];
+    return DPX_OK;
+}
+
+}  // namespace
+
+extern "C" int dpx_batch_run_noalloc(const uint8_t* d_in, uint8_t* d_out, const dpx_item* items, int n_items,
+                                     void* stream_)
+{
+    DPX_CHECK(d_in && d_out && items && n_items > 0, DPX_EINVAL, "bad arguments");
+    DPX_CHECK(n_items <= kNaMaxItems, DPX_EINVAL, "%d items: dpx_batch_run_noalloc takes at most %d", n_items,
+              kNaMaxItems);
+    hipStream_t stream = as_stream(stream_);
+    int dev = 0;
+    DPX_HIP(hipGetDevice(&dev));
+    DPX_CHECK(dev >= 0 && dev < kNaMaxDevices, DPX_ENODEV, "device %d", dev);
+    // pass 1: validate, graph bytes and tiles, per group (2 * scale + sub)
+    // its item and tile counts
+    uint32_t gcount[2 * (kMaxScale + 1)] = {}, gtiles[2 * (kMaxScale + 1)] = {};
+    size_t gbytes = 0, graph_tiles = 0, raster_tiles = 0;
+    for (int k = 0; k < n_items; k++) {
+        DevItem d;
+        if (int rc = make_dev_item(items[k], k, d_out, 0, d)) return rc;
+        DPX_CHECK(!d.subdivide, DPX_EINVAL,
+                  "item %d: subdivision needs per-cell row buffers; dpx_batch_run_noalloc takes items without it", k);
+        gbytes += align_up((size_t)d.W * d.H, 256);
+        graph_tiles += (size_t)((d.W + GTX - 1) / GTX) * ((d.H + GTY - 1) / GTY);
+        const size_t rt = (size_t)((d.W + kNaTX - 1) / kNaTX) * ((d.H + kNaTY - 1) / kNaTY);
+        raster_tiles += rt;
+        const int g = 2 * d.scale + d.subdivide;
+        gcount[g]++;
+        gtiles[g] += (uint32_t)std::min<size_t>(rt, 0xFFFFFFFFu);
+    }
+    DPX_CHECK(gbytes <= kNaGraphBytes, DPX_EINVAL,
+              "batch needs %zu graph bytes: dpx_batch_run_noalloc holds at most %zu (one byte per pixel, each item "
+              "rounded up to 256)", gbytes, kNaGraphBytes);
+    DPX_CHECK(graph_tiles <= kNaGraphTiles && raster_tiles <= kNaRasterTiles, DPX_EINVAL,
+              "batch has %zu graph and %zu raster tiles: dpx_batch_run_noalloc holds at most %zu and %zu", graph_tiles,
+              raster_tiles, kNaGraphTiles, kNaRasterTiles);
+    uint8_t* base = nullptr;
+    if (int rc = na_setup(dev, stream, &base)) return rc;
+    DevItem* d_items = reinterpret_cast<DevItem*>(base + kNaOffItems);
+    DevItem* d_gitems = reinterpret_cast<DevItem*>(base + kNaOffGroupItems);
+    uint2* d_gmap = reinterpret_cast<uint2*>(base + kNaOffGmap);
+    uint2* d_rmap = reinterpret_cast<uint2*>(base + kNaOffRmap);
+    uint8_t* d_graph = base + kNaOffGraph;
+    // groups in ascending (scale, subdivision): first item slot and tile
+    uint32_t gslot0[2 * (kMaxScale + 1)], gtile0[2 * (kMaxScale + 1)];
+    uint32_t slot = 0, tile = 0;
+    for (int g = 0; g < 2 * (kMaxScale + 1); g++) {
+        gslot0[g] = slot;
+        gtile0[g] = tile;
+        slot += gcount[g];
+        tile += gtiles[g];
+    }
+    // pass 2: descriptors to the device, kNaPutChunk items per launch
+    uint32_t gnext[2 * (kMaxScale + 1)] = {}, gtnext[2 * (kMaxScale + 1)] = {};
+    NaPutArgs args;
+    memset(&args, 0, sizeof(args));
+    uint64_t goff = 0;
+    uint32_t gt = 0;
+    for (int k = 0; k < n_items; k++) {
+        NaPut& p = args.it[args.n++];
+        (void)make_dev_item(items[k], k, d_out, goff, p.d);  // (validated in pass 1)
+        const int g = 2 * p.d.scale + p.d.subdivide;
+        p.index = (uint32_t)k;
+        p.gpos = gnext[g]++;
+        p.gslot = gslot0[g] + p.gpos;
+        p.gtile = gt;
+        p.rtile = gtile0[g] + gtnext[g];
+        gt += (uint32_t)(((p.d.W + GTX - 1) / GTX) * ((p.d.H + GTY - 1) / GTY));
+        gtnext[g] += (uint32_t)(((p.d.W + kNaTX - 1) / kNaTX) * ((p.d.H + kNaTY - 1) / kNaTY));
+        goff += align_up((size_t)p.d.W * p.d.H, 256);
+        if (args.n == kNaPutChunk || k == n_items - 1) {
+            hipLaunchKernelGGL(k_na_put, dim3(1), dim3(256), 0, stream, args, d_items, d_gitems, d_gmap, d_rmap);
+            args.n = 0;
+        }
+    }
+    hipLaunchKernelGGL(kf_graph, dim3((unsigned)graph_tiles), dim3(256), 0, stream, d_items, d_gmap, 0, d_in,
+                       d_graph);
+    for (int g = 0; g < 2 * (kMaxScale + 1); g++) {
+        if (!gcount[g]) continue;
+        const int S = g / 2;
+        RasterFn fn = pick_raster(S, g % 2, kNaTX);
+        DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", S);
+        hipLaunchKernelGGL(fn, dim3(gtiles[g]), dim3(256), 0, stream, d_gitems + gslot0[g], d_rmap + gtile0[g], 0,
+                           d_in, d_graph, nullptr, nullptr,
+                           reinterpret_cast<const uint32_t*>(base + na_mask_off(S)),
+                           reinterpret_cast<const uint16_t*>(base + na_info_off(S)), d_out);
+    }
+    DPX_HIP(hipGetLastError());
+    return DPX_OK;
+}
+

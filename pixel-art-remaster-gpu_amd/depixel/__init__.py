@@ -98,6 +98,7 @@ _SIGS = {
     "dpx_context_create": (c_int, [c_int, POINTER(c_void_p)]),
     "dpx_context_destroy": (None, [c_void_p]),
     "dpx_batch_run": (c_int, [c_void_p, c_void_p, c_void_p, POINTER(Item), c_int, c_void_p]),
+    "dpx_batch_run_noalloc": (c_int, [c_void_p, c_void_p, POINTER(Item), c_int, c_void_p]),
     "dpx_batch_graph": (c_int, [c_void_p, POINTER(c_void_p), POINTER(POINTER(c_uint64))]),
     "dpx_context_keep_graph": (c_int, [c_void_p, c_int]),
     "dpx_context_set_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),

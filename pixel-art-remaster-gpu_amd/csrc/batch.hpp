@@ -190,34 +190,12 @@ __device__ inline void tile_snapshot_load(const DevItem& it, const uint8_t* __re
 // positions per thread, into s_g32 (resolved rows y0 - 1 .. y0 + RTY), and
 // the tile's own resolved bytes into gfin when it is given.  s_p32 must
 // have been cleared and *s_nblk zeroed before the barrier.
-//
-// The curve walks of the decisions (up to 31 steps) leave the small window
-// on line-heavy frames; each step outside it is a dependent HBM read.  A
-// tile that has crossed blocks therefore first loads a larger walk window
-// of the snapshot into s_walk (an LDS buffer of at least kWalkBytes that the
-// kernel uses only later): kWalkEX more bytes on each side and kWalkEY more
-// rows above and below.  Tiles without crossings load nothing more.
-#ifndef DPX_WALK_WINDOW
-#define DPX_WALK_WINDOW 0
-#endif
-constexpr int kWalkEX = 32, kWalkEY = 24;
-template <int TX>
-struct WalkWindow {
-    static constexpr int B = TileShape<TX>::DB + 2 * kWalkEX;    // bytes per row (a multiple of 4)
-    static constexpr int R = TileShape<TX>::AY + 2 * kWalkEY;    // rows
-    static constexpr int BYTES = B * R;
-};
-constexpr int kWalkBytes = 8192;  // room the callers provide (the largest shape's window fits)
-static_assert(WalkWindow<16>::BYTES <= kWalkBytes && WalkWindow<32>::BYTES <= kWalkBytes &&
-                  WalkWindow<64>::BYTES <= kWalkBytes, "walk windows fit their buffer");
-
 template <int TX>
 __device__ inline void tile_resolve(const DevItem& it, const uint8_t* __restrict__ aux, uint8_t* __restrict__ gfin,
                                     int x0, int y0, int ay0, const uint32_t* s_aux32, uint32_t* s_p32,
-                                    uint16_t* s_blk, int* s_nblk, uint32_t* s_g32, uint32_t* s_walk, int t)
+                                    uint16_t* s_blk, int* s_nblk, uint32_t* s_g32, int t)
 {
     DPX_TILE_NAMES(TX);
-    using WW_ = WalkWindow<TX>;
     for (int k = t; k < (GY + 1) * RDW; k += 256) {
         const int r = k / RDW, w = k % RDW + 1;  // word w of snapshot row r: origins of blocks on rows r, r + 1
         const uint32_t* a = s_aux32 + r * RWW + w;
@@ -231,45 +209,14 @@ __device__ inline void tile_resolve(const DevItem& it, const uint8_t* __restrict
     }
     __syncthreads();
     const int nblk = *s_nblk;
-    const bool walk_win = DPX_WALK_WINDOW && nblk > 0;  // uniform over the block
-    const int ex0 = x0 - 4 - kWalkEX, ey0 = ay0 - kWalkEY;  // the walk window's first byte and row
-    if (walk_win) {
-        if (it.graph_w4) {  // x0 and kWalkEX are multiples of 4: whole words inside or outside the frame
-            for (int k = t; k < WW_::R * (WW_::B / 4); k += 256) {
-                const int r = k / (WW_::B / 4), w = k % (WW_::B / 4);
-                const int x = ex0 + 4 * w, y = ey0 + r;
-                uint32_t v = 0;
-                if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
-                    v = *reinterpret_cast<const uint32_t*>(aux + (uint32_t)(y * it.W + x));
-                s_walk[k] = v;
-            }
-        } else {
-            uint8_t* s8 = reinterpret_cast<uint8_t*>(s_walk);
-            for (int k = t; k < WW_::BYTES; k += 256) {
-                const int x = ex0 + k % WW_::B, y = ey0 + k / WW_::B;
-                s8[k] = ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
-                            ? aux[(uint32_t)(y * it.W + x)] : 0;
-            }
-        }
-        __syncthreads();
-    }
     {
         // (the window is read by words, so that the compiler keeps the LDS
         // and HBM reads apart instead of merging them into one generic load)
         auto get = [&](int gx, int gy) -> uint32_t {
-            bool win;
-            uint32_t v;
-            if (walk_win) {
-                const int wx = gx - ex0, wy = gy - ey0;
-                win = (unsigned)wx < (unsigned)WW_::B && (unsigned)wy < (unsigned)WW_::R;
-                const int c = win ? wy * WW_::B + wx : 0;
-                v = (s_walk[c >> 2] >> (8 * (c & 3))) & 0xFFu;
-            } else {
-                const int wx = gx - (x0 - 4), wy = gy - ay0;
-                win = (unsigned)wx < (unsigned)RDB && (unsigned)wy < (unsigned)AY;
-                const int c = win ? wy * RWB + wx + 4 : 0;
-                v = (s_aux32[c >> 2] >> (8 * (c & 3))) & 0xFFu;
-            }
+            int wx = gx - (x0 - 4), wy = gy - ay0;
+            bool win = (unsigned)wx < (unsigned)RDB && (unsigned)wy < (unsigned)AY;
+            int c = win ? wy * RWB + wx + 4 : 0;
+            uint32_t v = (s_aux32[c >> 2] >> (8 * (c & 3))) & 0xFFu;
             if (!win) v = aux[(uint32_t)(gy * it.W + gx)];
             return v;
         };
@@ -384,7 +331,7 @@ __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ 
     if (t < (GY + 1) * RWW) s_p32[t] = 0;
     tile_snapshot_load<TX>(it, aux, x0, ay0, s_aux32, t);
     __syncthreads();
-    tile_resolve<TX>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, s_poly, t);
+    tile_resolve<TX>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, t);
     __syncthreads();
     // hulls of the own cells and of their neighbours: cell (x0 - 1 + lx,
     // y0 - 1 + ly) at resolved row ly, byte lx + 7
@@ -530,7 +477,6 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     constexpr bool CODES = S >= 6;
     constexpr int PX_WORDS = 2048, PX_CELLS = PX_WORDS / (S * S), PX_CODE_CELLS = 4 * PX_WORDS / (S * S);
     static_assert(PX_CELLS >= 1, "at least one buffered cell");
-    static_assert(PX_WORDS * 4 >= kWalkBytes, "the pixel buffer holds the walk window before it holds pixels");
     __shared__ uint32_t s_px[PX_WORDS];
     uint8_t* s_px8 = reinterpret_cast<uint8_t*>(s_px);
     __shared__ uint16_t s_idx[RTX * RTY];    // list index of a listed own cell
@@ -582,7 +528,7 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     }
     __syncthreads();
     if constexpr (!SUB) {
-        tile_resolve<TX>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, s_px, t);
+        tile_resolve<TX>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, t);
         __syncthreads();
     }
     // candidate cells: key and coverage summary.  Candidate (lx, ly) is the

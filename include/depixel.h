@@ -147,10 +147,15 @@ int dpx_stage_components(const uint8_t* d_graph, int width, int height, int32_t*
 /* border chains of a resolved graph: extractBorderPoints cc_functions.cu:348-503
  * (its CClist / CCsizes outputs).  Chain k is d_nodes[sum of d_sizes[0..k-1] ..]
  * of d_sizes[k] node indices j*width+i, chains in the order the row-major scan
- * starts them.  d_scratch: width*height bytes.  d_counts[3] receives the
- * number of chains, of nodes and a status (0; DPX_ENOMEM when max_nodes or
- * max_chains is too small; DPX_EINVAL when the graph is not a resolved,
- * symmetric similarity graph), readable once the stream reaches this point. */
+ * starts them.  d_scratch: width*height bytes, 4-byte aligned (it holds the
+ * scan's marks when they do not fit a block's LDS, frames of more than about
+ * 650,000 nodes).  Working memory of about 300 bytes per node is allocated
+ * and freed in stream order (hipMallocAsync / hipFreeAsync on `stream`).
+ * d_counts[3] receives the number of chains, of nodes and a status (0;
+ * DPX_ENOMEM when max_nodes or max_chains is too small; DPX_EINVAL, with no
+ * chains, when a link leaves the frame or has no reverse link: the graph
+ * is not a resolved, symmetric similarity graph), readable once the stream
+ * reaches this point. */
 int dpx_stage_border_chains(const uint8_t* d_graph, int width, int height, uint8_t* d_scratch,
                             int32_t* d_nodes, int max_nodes, int32_t* d_sizes, int max_chains,
                             int32_t* d_counts, void* stream);

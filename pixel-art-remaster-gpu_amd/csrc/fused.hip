@@ -180,11 +180,21 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
         s_h32[t] = lw;
         __syncthreads();
     }
-    // Lane word of pixel (x, y) at byte offset `off` of the item's buffer.
+    // Lane word of pixel (x, y) at byte offset `off` of the item's buffer:
+    // one 4-byte load (any alignment) where the 4 bytes lie inside the
+    // item's rows, else byte loads (the last pixel of the last row in
+    // memory of a 3-channel frame without row padding).
+    const uint32_t in_end = (uint32_t)(it.H - 1) * (uint32_t)it.stride + (uint32_t)(it.W * it.ch);
     auto lane_word = [&](uint32_t off) -> uint32_t {
         if (indexed) return s_h32[base[off]];
-        int c0 = base[off + o0], c1 = base[off + 1], c2 = base[off + o2];
-        if (alpha && base[off + 3] == 0) return kLaneTransparent;
+        uint32_t w;
+        if (off + 4u <= in_end)
+            __builtin_memcpy(&w, base + off, 4);
+        else
+            w = base[off] | (uint32_t)base[off + 1] << 8 | (uint32_t)base[off + 2] << 16 |
+                (it.ch == 4 ? (uint32_t)base[off + 3] << 24 : 0u);
+        const int c0 = (w >> (8 * o0)) & 0xFF, c1 = (w >> 8) & 0xFF, c2 = (w >> (8 * o2)) & 0xFF;
+        if (alpha && (w >> 24) == 0) return kLaneTransparent;
         return yuv_lanes(yuv_pack(c0, c1, c2)) | kLaneOpaque;
     };
     // Window column c (x = x0 - 1 + c), row r (y = y0 - 2 + r).  Columns

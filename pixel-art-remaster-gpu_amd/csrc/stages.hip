@@ -8,6 +8,8 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <utility>
+
 #include "common.hpp"
 #include "geom.hpp"
 #include "graph.hpp"
@@ -531,103 +533,341 @@ __global__ void k_cc_final(int32_t* parent, int32_t* labels, int N)
     if (k < N) labels[k] = uf_root(parent, k);
 }
 
-// Border chains (extractBorderPoints cc_functions.cu:348-503) of one frame,
-// one 64-lane block.  Which walks become chains depends on every earlier
-// walk (a walk is abandoned on a node an earlier chain listed or an earlier
-// abandoned walk discarded), so the scan is sequential by definition: the
-// lanes clear the marks and find start candidates 64 nodes at a time, and
-// lane 0 walks.  A walk starts at a node that is neither internal (90) nor
-// without links nor listed, leaves by its first link clockwise from
-// up-left, and at each node turns clockwise from the link it came in by.
-// It is a chain when it comes back to its start through the link just
-// counter-clockwise of the first one; it is abandoned on an internal,
-// listed or discarded node, discarding the node it stepped from.  Chain
-// nodes are listed; an abandoned walk leaves its start unlisted (the
+// Border chains (extractBorderPoints cc_functions.cu:348-503) of one frame.
+// The scan: a walk starts at each node, in row-major order, that is neither
+// internal (90) nor without links nor listed; it leaves by the node's first
+// link clockwise from up-left and at each node turns clockwise from the link
+// it came in by.  It is a chain when it comes back to its start through the
+// link just counter-clockwise of the first one; it is abandoned on an
+// internal, listed or discarded node, discarding the node it stepped from.
+// Chain nodes are listed; an abandoned walk leaves its start unlisted (the
 // behaviour that reproduces the chains recorded in alex_png.txt:1-21).
-// mark: 0 free, 1 listed, 2 discarded.  counts: chains, nodes, status
-// (0, or DPX_ENOMEM when the output arrays are too small, or DPX_EINVAL when
-// a walk leaves the frame or does not close: the graph is not a resolved,
-// symmetric similarity graph).
 //
-// The walk is a chain of dependent reads (each step's node decides the
-// next), so its speed is the latency of those reads: with LDS the frame's
-// graph bytes and marks are copied into the block's LDS first (2 bytes per
-// node, frames that fit), otherwise they are read from HBM (L2).
-template <bool LDS>
-__global__ __launch_bounds__(64) void k_border_chains(const uint8_t* __restrict__ graph_g, int W, int H,
-                                                      uint8_t* mark_g, int32_t* nodes, int32_t* sizes, int max_nodes,
-                                                      int max_chains, int32_t* counts)
+// Which walks become chains depends on every earlier walk, but where a walk
+// goes does not: the turn rule gives each directed link (node s, link e) one
+// successor (the neighbour t, its first link clockwise after 7 - e), and on
+// a symmetric graph that is a permutation of the directed links whose
+// cycles are the faces of the graph's plane embedding.  A walk from s runs
+// along the cycle of s's first link and closes where that cycle comes back
+// to it; a link into an internal node has no successor (the walk stops
+// there), which cuts some cycles into paths.  So the cycles and paths are
+// laid out first, all links in parallel: list ranking by pointer jumping
+// (Wyllie) finds each link's distance to the end of its path and, on a
+// cycle, the smallest link of the cycle, where the cycle is then cut; each
+// path is written as an array of its links' nodes in walk order.  The scan
+// itself (one wave) then needs, per walk, only the marks of the nodes of a
+// contiguous range of that array, 64 at a time.
+//
+// Directed link id = node * 8 + link.  State per link (uint4): x = next
+// link 2^k steps on (kEnd past the end, kNoLink for slots that are not
+// links of a walkable node), y = steps to it, z = the furthest link reached
+// (the path's last link once x = kEnd), w = the smallest link seen (cycle
+// search) or flags (bit 0: on a cycle, bit 1: first link of its path).
+constexpr int kEnd = -1, kNoLink = -2;
+
+inline size_t round256(size_t v) { return (v + 255) / 256 * 256; }
+
+// successor of directed link (s, e) of a walkable node, or kEnd when the
+// neighbour is internal
+__device__ inline int bc_succ(const uint8_t* g, int W, int s, int e)
 {
-    extern __shared__ uint8_t s_dyn[];
-    __shared__ int s_status;
-    const int N = W * H, lane = threadIdx.x;
-    const uint8_t* graph = graph_g;
-    uint8_t* mark = mark_g;
-    if constexpr (LDS) {
-        uint8_t* g = s_dyn;
-        for (int k = lane; k < N; k += 64) g[k] = graph_g[k];
-        graph = g;
-        mark = s_dyn + N;
+    const int t = s + edy(e) * W + edx(e);
+    const uint32_t vt = g[t];
+    return vt == 90u ? kEnd : t * 8 + cw_turn(vt, 7 - e, +1);
+}
+
+__global__ void k_bc_clear(int* ctr)
+{
+    if (threadIdx.x < 4) ctr[threadIdx.x] = 0;
+}
+
+// ctr[0]: status (DPX_EINVAL: a link leaves the frame or has no reverse
+// link), ctr[1]: entries of the path arrays handed out
+__global__ __launch_bounds__(256) void k_bc_init(const uint8_t* __restrict__ g, int W, int H, uint4* __restrict__ S,
+                                                 int* ctr)
+{
+    const uint32_t id = blockIdx.x * 256u + threadIdx.x, E = 8u * (uint32_t)W * (uint32_t)H;
+    if (id >= E) return;
+    const int s = (int)(id >> 3), e = (int)(id & 7u);
+    const uint32_t v = g[s];
+    if (!((v >> e) & 1u) || v == 90u) {
+        S[id] = make_uint4((uint32_t)kNoLink, 0u, id, 0u);
+        return;
     }
-    for (int k = lane; k < N; k += 64) mark[k] = 0;
-    if (lane == 0) s_status = 0;
+    const int ni = s % W + edx(e), nj = s / W + edy(e);
+    if (!in_frame(ni, nj, W, H) || !((g[nj * W + ni] >> (7 - e)) & 1u)) {
+        ctr[0] = DPX_EINVAL;
+        S[id] = make_uint4((uint32_t)kNoLink, 0u, id, 0u);
+        return;
+    }
+    const int nx = bc_succ(g, W, s, e);
+    S[id] = make_uint4((uint32_t)nx, nx >= 0 ? 1u : 0u, nx >= 0 ? (uint32_t)nx : id, id);
+}
+
+// one pointer-jumping round, in -> out; MIN: also the smallest link seen
+template <bool MIN>
+__global__ __launch_bounds__(256) void k_bc_jump(const uint4* __restrict__ in, uint4* __restrict__ out, uint32_t E)
+{
+    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= E) return;
+    uint4 st = in[id];
+    if ((int)st.x >= 0) {
+        const uint4 x = in[st.x];
+        st = make_uint4(x.x, st.y + x.y, x.z, MIN ? (x.w < st.w ? x.w : st.w) : st.w);
+    }
+    out[id] = st;
+}
+
+// after the cycle search: links still jumping lie on cycles; each cycle is
+// cut before its smallest link, which becomes the first of its path.  A
+// link on a path is the first of its path when its predecessor (the link
+// that turns into it: from the neighbour across s's link just
+// counter-clockwise of e) is no walkable node's link.
+__global__ __launch_bounds__(256) void k_bc_cut(const uint8_t* __restrict__ g, int W, int H,
+                                                const uint4* __restrict__ F, uint4* __restrict__ G, uint32_t E)
+{
+    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= E) return;
+    const uint4 st = F[id];
+    if ((int)st.x == kNoLink) {
+        G[id] = st;
+        return;
+    }
+    const int s = (int)(id >> 3), e = (int)(id & 7u);
+    if ((int)st.x >= 0) {
+        const int nx = bc_succ(g, W, s, e);
+        const uint32_t fl = 1u | (id == st.w ? 2u : 0u);
+        G[id] = (uint32_t)nx == st.w ? make_uint4((uint32_t)kEnd, 0u, id, fl) : make_uint4((uint32_t)nx, 1u, (uint32_t)nx, fl);
+        return;
+    }
+    const int a = cw_turn(g[s], e, -1);
+    const int pi = s % W + edx(a), pj = s / W + edy(a);
+    const bool first = !in_frame(pi, pj, W, H) || g[pj * W + pi] == 90u;
+    G[id] = make_uint4((uint32_t)kEnd, st.y, st.z, first ? 2u : 0u);
+}
+
+// the first link of each path takes the path's range of the node array:
+// info[last link] = (offset, length, on a cycle)
+__global__ __launch_bounds__(256) void k_bc_heads(const uint4* __restrict__ F, uint4* __restrict__ info, int* ctr,
+                                                  uint32_t E)
+{
+    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= E || ctr[0] != 0) return;
+    const uint4 st = F[id];
+    if ((int)st.x == kNoLink) return;
+    if ((int)st.x != kEnd) {  // not reached the end: the graph is not a permutation of links
+        ctr[0] = DPX_EINVAL;
+        return;
+    }
+    if (!(st.w & 2u)) return;
+    const uint32_t len = st.y + 1u;
+    const uint32_t off = (uint32_t)atomicAdd(&ctr[1], (int)len);
+    if ((uint64_t)off + len > E) {
+        ctr[0] = DPX_EINVAL;
+        return;
+    }
+    info[st.z] = make_uint4(off, len, st.w & 1u, 0u);
+}
+
+// each link writes its node at its place in its path's array; the first
+// link of a walkable node gives that node's walk (offset, length, place of
+// the start, on a cycle)
+__global__ __launch_bounds__(256) void k_bc_scatter(const uint8_t* __restrict__ g, const uint4* __restrict__ F,
+                                                    const uint4* __restrict__ info, int32_t* __restrict__ A,
+                                                    uint4* __restrict__ rec, const int* ctr, uint32_t E)
+{
+    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    if (id >= E || ctr[0] != 0) return;
+    const uint4 st = F[id];
+    if ((int)st.x == kNoLink) return;
+    const uint4 inf = info[st.z];
+    if (st.y >= inf.y || (uint64_t)inf.x + inf.y > E) return;  // (inconsistent only for graphs refused above)
+    const uint32_t pos = inf.y - 1u - st.y;
+    A[inf.x + pos] = (int32_t)(id >> 3);
+    if ((int)(id & 7u) == cw_first_link(g[id >> 3])) rec[id >> 3] = make_uint4(inf.x, inf.y, pos, inf.z);
+}
+
+// The scan, one wave (wave 0 of the block).  Marks are two bitmaps
+// (listed, discarded) of (N + 31) / 32 words each, in LDS when they fit,
+// else in bits_g.  Its time is the latency of dependent steps, so what does
+// not depend on the marks is fetched ahead: the other three waves read the
+// path arrays and the graph once, so that the scan's reads of them hit the
+// L2 of its XCD; the next two groups' records are loaded while a group is
+// scanned; the first kBcChunk nodes of the next kBcAhead walks are loaded at
+// once (a walk's nodes do not depend on the marks, only its outcome does).
+// A walk's nodes are tested kBcChunk at a time (kBcU per lane).
+constexpr int kBcU = 4, kBcChunk = 64 * kBcU, kBcAhead = 4;
+
+__device__ inline int lane_of(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+template <bool LDS>
+__global__ __launch_bounds__(256) void k_bc_scan(const uint8_t* __restrict__ g, int N, const uint4* __restrict__ rec,
+                                                 const int32_t* __restrict__ A, uint32_t* bits_g, int32_t* nodes,
+                                                 int32_t* sizes, int max_nodes, int max_chains, const int* ctr,
+                                                 int32_t* counts)
+{
+    extern __shared__ uint32_t s_bits[];
+    __shared__ int32_t s_pre[kBcAhead][kBcChunk];
+    uint32_t* bits = LDS ? s_bits : bits_g;
+    const int NW = (N + 31) >> 5, tid = threadIdx.x, lane = tid & 63;
+    uint32_t* listed = bits;
+    uint32_t* disc = bits + NW;
+    for (int k = tid; k < 2 * NW; k += 256) bits[k] = 0u;
+    if constexpr (!LDS) __threadfence();
     __syncthreads();
-    int nc = 0, nn = 0;  // lane 0's totals
-    for (int base = 0; base < N; base += 64) {
+    if (tid >= 64) {  // the helper waves: touch the path arrays and the graph, then leave
+        const volatile int32_t* vA = A;
+        const volatile uint8_t* vg = g;
+        const int total = ctr[0] == 0 ? ctr[1] : 0;
+        for (int k = (tid - 64) * 16; k < total; k += 192 * 16) (void)vA[k];  // one read per 64 bytes
+        for (int k = (tid - 64) * 64; k < N; k += 192 * 64) (void)vg[k];
+        return;
+    }
+    auto get = [&](const uint32_t* b, int n) -> uint32_t {
+        uint32_t w;
+        if constexpr (LDS)
+            w = b[n >> 5];
+        else
+            w = __hip_atomic_load(b + (n >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return (w >> (n & 31)) & 1u;
+    };
+    auto put = [&](uint32_t* b, int n) { atomicOr(b + (n >> 5), 1u << (n & 31)); };
+    // the marks written so far are seen by the wave's next reads (one wave:
+    // its LDS operations complete in order)
+    auto settle = [&]() {
+        if constexpr (!LDS) __threadfence();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    };
+    // node k (1-based, k <= n) of a walk from place pos of a path of L
+    auto walk_node = [&](int off, int L, int pos, int k) {
+        int q = pos + k;
+        if (q >= L) q -= L;
+        return A[off + q];
+    };
+    int status = ctr[0], nc = 0, nn = 0;
+    uint32_t v1 = lane < N ? g[lane] : 90u, v2 = 64 + lane < N ? g[64 + lane] : 90u;
+    uint4 r1 = lane < N ? rec[lane] : make_uint4(0u, 0u, 0u, 0u);
+    uint4 r2 = 64 + lane < N ? rec[64 + lane] : make_uint4(0u, 0u, 0u, 0u);
+    for (int base = 0; base < N && status == 0; base += 64) {
         const int me = base + lane;
-        const uint32_t mv = me < N ? graph[me] : 90u;
-        uint64_t cand = __ballot(mv != 90u && mv != 0u);
-        if (lane == 0) {
-            while (cand && s_status == 0) {
-                const int s = base + __builtin_ctzll(cand);
-                cand &= cand - 1;
-                if (mark[s] == 1) continue;
-                const uint32_t v = graph[s];
-                int e = cw_first_link(v);
-                const int arrive = cw_turn(v, e, -1);
-                int ci = s % W, cj = s / W, cur = s, last = s;
-                long len = 1;
-                bool kept = true;
-                if (nn < max_nodes) nodes[nn] = s;
-                for (;;) {
-                    const int ni = ci + edx(e), nj = cj + edy(e);
-                    if (!in_frame(ni, nj, W, H) || len > 8L * N) {
-                        s_status = DPX_EINVAL;
-                        break;
-                    }
-                    const int nxt = nj * W + ni;
-                    if (nxt == s && 7 - e == arrive) break;  // back at the start: a chain
-                    const uint32_t nv = graph[nxt];
-                    if (nv == 90u || mark[nxt] != 0) {
-                        mark[last] = 2;
-                        kept = false;
-                        break;
-                    }
-                    e = cw_turn(nv, 7 - e, +1);
-                    ci = ni;
-                    cj = nj;
-                    cur = nxt;
-                    if (nn + len < max_nodes) nodes[nn + len] = cur;
-                    len++;
-                    last = cur;
+        const uint32_t v = v1;
+        const uint4 r = r1;
+        v1 = v2;
+        r1 = r2;
+        if (me + 128 < N) {  // (a record of a node that starts no walk is never used)
+            v2 = g[me + 128];
+            r2 = rec[me + 128];
+        }
+        const bool cand = v != 90u && v != 0u;
+        uint64_t m = __ballot(cand && !get(listed, me));
+        while (m && status == 0) {
+            // the next walks of the group: their first chunks, all loads at once
+            int bs[kBcAhead];
+#pragma unroll
+            for (int a = 0; a < kBcAhead; a++) {
+                bs[a] = m ? __builtin_ctzll(m) : -1;
+                m &= m ? m - 1 : 0;
+            }
+#pragma unroll
+            for (int a = 0; a < kBcAhead; a++) {
+                if (bs[a] < 0) break;
+                const int off = lane_of((int)r.x, bs[a]), L = lane_of((int)r.y, bs[a]), pos = lane_of((int)r.z, bs[a]);
+                const int n = lane_of((int)r.w, bs[a]) ? L - 1 : L - 1 - pos;
+#pragma unroll
+                for (int u = 0; u < kBcU; u++) {
+                    const int k = 1 + lane + 64 * u;
+                    s_pre[a][lane + 64 * u] = k <= n ? walk_node(off, L, pos, k) : 0;
                 }
-                if (s_status || !kept) continue;
-                if (nc >= max_chains || nn + len > max_nodes) {
-                    s_status = DPX_ENOMEM;
+            }
+            uint64_t alive = ~0ull;  // candidates not listed by a chain of this batch
+            for (int a = 0; a < kBcAhead && status == 0; a++) {
+                const int b = bs[a];
+                if (b < 0) break;
+                if (!((alive >> b) & 1u)) continue;
+                const int s = base + b;
+                const int off = lane_of((int)r.x, b), L = lane_of((int)r.y, b), pos = lane_of((int)r.z, b);
+                const bool cyc = lane_of((int)r.w, b) != 0;
+                const int n = cyc ? L - 1 : L - 1 - pos;  // walk nodes after the start
+                int prev = s, nd[kBcU];
+                bool hit = false;
+                for (int c = 0; c < n; c += kBcChunk) {
+                    uint64_t bal[kBcU];
+#pragma unroll
+                    for (int u = 0; u < kBcU; u++) {
+                        const int k = c + 1 + lane + 64 * u;
+                        nd[u] = k <= n ? (c == 0 ? s_pre[a][lane + 64 * u] : walk_node(off, L, pos, k)) : 0;
+                    }
+#pragma unroll
+                    for (int u = 0; u < kBcU; u++) {
+                        const int k = c + 1 + lane + 64 * u;
+                        bal[u] = __ballot(k <= n && (get(listed, nd[u]) | get(disc, nd[u])));
+                    }
+                    // place in the chunk of the node before the first marked
+                    // one (-1: the node before the chunk), else of the last
+                    uint64_t any = 0;
+#pragma unroll
+                    for (int u = 0; u < kBcU; u++) any |= bal[u];
+                    hit = any != 0;
+                    int last = (n - c < kBcChunk ? n - c : kBcChunk) - 1;
+                    if (hit) {
+                        last = -1;
+#pragma unroll
+                        for (int u = kBcU - 1; u >= 0; u--)
+                            if (bal[u]) last = 64 * u + __builtin_ctzll(bal[u]) - 1;
+                    }
+                    if (last >= 0) {
+                        int x = 0;
+#pragma unroll
+                        for (int u = 0; u < kBcU; u++)
+                            if ((last >> 6) == u) x = lane_of(nd[u], last & 63);
+                        prev = x;
+                    }
+                    if (hit) break;
+                }
+                if (hit || !cyc) {  // abandoned (a path ends at an internal node)
+                    if (lane == 0) put(disc, prev);
+                    settle();
                     continue;
                 }
-                for (long k = 0; k < len; k++) mark[nodes[nn + k]] = 1;
-                sizes[nc++] = (int32_t)len;
-                nn += (int)len;
+                if (nc >= max_chains || (int64_t)nn + L > max_nodes) {
+                    status = DPX_ENOMEM;
+                    break;
+                }
+                if (n < kBcChunk) {  // the whole walk is in nd
+                    if (lane == 0) {
+                        nodes[nn] = s;
+                        put(listed, s);
+                    }
+#pragma unroll
+                    for (int u = 0; u < kBcU; u++) {
+                        const int k = 1 + lane + 64 * u;
+                        if (k <= n) {
+                            nodes[nn + k] = nd[u];
+                            put(listed, nd[u]);
+                        }
+                    }
+                } else {
+                    for (int k = lane; k < L; k += 64) {
+                        const int node = k == 0 ? s : walk_node(off, L, pos, k);
+                        nodes[nn + k] = node;
+                        put(listed, node);
+                    }
+                }
+                if (lane == 0) sizes[nc] = L;
+                nc++;
+                nn += L;
+                settle();
+                const uint64_t still = __ballot(cand && !get(listed, me));
+                alive &= still;
+                m &= still;
             }
         }
-        __syncthreads();
-        if (s_status) break;
     }
     if (lane == 0) {
         counts[0] = nc;
         counts[1] = nn;
-        counts[2] = s_status;
+        counts[2] = status;
     }
 }
 
@@ -791,20 +1031,64 @@ extern "C" int dpx_stage_border_chains(const uint8_t* d_graph, int W, int H, uin
               "bad arguments");
     DPX_CHECK((int64_t)W * H <= (1 << 28), DPX_EINVAL, "frame too large (%dx%d)", W, H);
     DPX_CHECK(max_nodes >= 0 && max_chains >= 0, DPX_EINVAL, "negative capacity");
-    // frames whose graph and marks fit the block's LDS walk there
+    const hipStream_t s = as_stream(stream);
+    const uint32_t N = (uint32_t)W * (uint32_t)H, E = 8u * N;
+    // the scan's marks in LDS when they fit a block, else in d_scratch
     int dev = 0, lds_max = 0;
     DPX_HIP(hipGetDevice(&dev));
     DPX_HIP(hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
-    const size_t lds = 2 * (size_t)W * H;
-    if (lds + 1024 <= (size_t)lds_max &&
-        hipFuncSetAttribute(reinterpret_cast<const void*>(k_border_chains<true>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess) {
-        hipLaunchKernelGGL(k_border_chains<true>, dim3(1), dim3(64), lds, as_stream(stream), d_graph, W, H, d_scratch,
-                           d_nodes, d_sizes, max_nodes, max_chains, d_counts);
-    } else {
+    const size_t lds = 8 * (size_t)((N + 31) / 32);
+    hipFuncAttributes fa{};
+    DPX_HIP(hipFuncGetAttributes(&fa, reinterpret_cast<const void*>(k_bc_scan<true>)));
+    bool use_lds = lds + fa.sharedSizeBytes <= (size_t)lds_max &&
+                   hipFuncSetAttribute(reinterpret_cast<const void*>(k_bc_scan<true>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
+    if (!use_lds) {
         (void)hipGetLastError();
-        hipLaunchKernelGGL(k_border_chains<false>, dim3(1), dim3(64), 0, as_stream(stream), d_graph, W, H, d_scratch,
-                           d_nodes, d_sizes, max_nodes, max_chains, d_counts);
+        DPX_CHECK(reinterpret_cast<uintptr_t>(d_scratch) % 4 == 0, DPX_EINVAL,
+                  "d_scratch must be 4-byte aligned for a frame of %dx%d", W, H);
+    }
+    int rounds = 0;
+    while ((1ull << rounds) < E) rounds++;
+    // working memory, stream-ordered: two link-state arrays, the path
+    // arrays, the per-node walk records and the counters
+    const size_t szS = round256((size_t)E * sizeof(uint4)), szA = round256((size_t)E * 4),
+                 szR = round256((size_t)N * sizeof(uint4));
+    uint8_t* work = nullptr;
+    DPX_HIP(hipMallocAsync(reinterpret_cast<void**>(&work), 2 * szS + szA + szR + 256, s));
+    struct FreeAfter {  // freed in stream order on every return below
+        void* p;
+        hipStream_t s;
+        ~FreeAfter() { (void)hipFreeAsync(p, s); }
+    } free_after{ work, s };
+    uint4* SA = reinterpret_cast<uint4*>(work);
+    uint4* SB = reinterpret_cast<uint4*>(work + szS);
+    int32_t* A = reinterpret_cast<int32_t*>(work + 2 * szS);
+    uint4* rec = reinterpret_cast<uint4*>(work + 2 * szS + szA);
+    int* ctr = reinterpret_cast<int*>(work + 2 * szS + szA + szR);
+    const dim3 grid((E + 255) / 256);
+    hipLaunchKernelGGL(k_bc_clear, dim3(1), dim3(64), 0, s, ctr);
+    hipLaunchKernelGGL(k_bc_init, grid, dim3(256), 0, s, d_graph, W, H, SA, ctr);
+    uint4 *cur = SA, *oth = SB;
+    for (int r = 0; r < rounds; r++) {
+        hipLaunchKernelGGL(k_bc_jump<true>, grid, dim3(256), 0, s, cur, oth, E);
+        std::swap(cur, oth);
+    }
+    hipLaunchKernelGGL(k_bc_cut, grid, dim3(256), 0, s, d_graph, W, H, cur, oth, E);
+    std::swap(cur, oth);
+    for (int r = 0; r < rounds; r++) {
+        hipLaunchKernelGGL(k_bc_jump<false>, grid, dim3(256), 0, s, cur, oth, E);
+        std::swap(cur, oth);
+    }
+    hipLaunchKernelGGL(k_bc_heads, grid, dim3(256), 0, s, cur, oth, ctr, E);
+    hipLaunchKernelGGL(k_bc_scatter, grid, dim3(256), 0, s, d_graph, cur, oth, A, rec, ctr, E);
+    if (use_lds) {
+        hipLaunchKernelGGL(k_bc_scan<true>, dim3(1), dim3(256), lds, s, d_graph, (int)N, rec, A, nullptr, d_nodes,
+                           d_sizes, max_nodes, max_chains, ctr, d_counts);
+    } else {
+        hipLaunchKernelGGL(k_bc_scan<false>, dim3(1), dim3(256), 0, s, d_graph, (int)N, rec, A,
+                           reinterpret_cast<uint32_t*>(d_scratch), d_nodes, d_sizes, max_nodes, max_chains, ctr,
+                           d_counts);
     }
     DPX_HIP(hipGetLastError());
     return DPX_OK;

@@ -420,15 +420,33 @@ def test_border_chains_match_oracle(gpu, D, O):
     print("alex", flush=True)
     got, nn, status = gpu_border_chains(torch, D, alex)
     assert status == 0 and got == expected and nn == sum(map(len, expected))
-    # (320x288 needs more LDS than a block has: its walks read HBM)
+    # (1024x700: the scan's marks do not fit a block's LDS and live in d_scratch)
     for kind, seed, W, H in ((0, 31, 256, 224), (2, 32, 97, 61), (4, 33, 120, 90), (0, 34, 1, 1), (2, 35, 7, 1),
-                             (2, 36, 320, 288)):
+                             (2, 36, 320, 288), (4, 37, 640, 480), (0, 38, 1024, 700), (2, 39, 33, 1),
+                             (2, 40, 1, 40)):
         print("frame", kind, seed, W, H, flush=True)
         img = D.synth_frame(kind, seed, W, H, 3)
         g = O.heuristic_cross(O.trivial_cross(O.similarity_graph(img, W, H)))
         got, nn, status = gpu_border_chains(torch, D, g)
         assert status == 0
         assert got == O.border_chains(g), f"frame {kind}/{seed}"
+
+
+def test_border_chains_refuse_invalid_graphs(gpu, D, O):
+    """A link that leaves the frame or has no reverse link: status
+    DPX_EINVAL, no chains."""
+    torch = gpu
+    g = np.zeros((5, 6), dtype=np.uint8)
+    g[0, 0] = 1 << 0  # up-left of (0, 0): outside the frame
+    _, nn, status = gpu_border_chains(torch, D, g)
+    assert status == D.EINVAL and nn == 0
+    g = np.zeros((5, 6), dtype=np.uint8)
+    g[2, 2] = 1 << 4  # right, without the left link back
+    _, nn, status = gpu_border_chains(torch, D, g)
+    assert status == D.EINVAL and nn == 0
+    g[2, 3] = 1 << 3  # now symmetric: one two-node chain
+    chains, nn, status = gpu_border_chains(torch, D, g)
+    assert status == 0 and chains == O.border_chains(g)
 
 
 def test_border_chains_report_small_outputs(gpu, D, O):

@@ -85,7 +85,7 @@ def test_random_frame_stages_match_oracle(gpu, D, O, seed):
     """The plain per-frame path on a random frame (kind, size 1..120 x 1..90,
     scale 1..8, subdivision, byte and row order): pixels, graph, edge counts
     and the defined slots of the triangle diagram equal the oracle's, and so
-    do the component labels of its graph."""
+    do the component labels and the border chains of its graph."""
     torch = gpu
     rng = np.random.default_rng(9000 + seed)
     kind = int(rng.choice([0, 1, 2, 4]))
@@ -111,6 +111,9 @@ def test_random_frame_stages_match_oracle(gpu, D, O, seed):
     D.check(D.lib.dpx_stage_components(d_g.data_ptr(), W, H, d_lab.data_ptr(), s.cuda_stream))
     torch.cuda.synchronize()
     assert np.array_equal(d_lab.cpu().numpy().reshape(H, W), O.components(ref["graph"])), what + ": labels"
+    from test_gpu_parity import gpu_border_chains
+    chains, nn, status = gpu_border_chains(torch, D, ref["graph"])
+    assert status == 0 and chains == O.border_chains(ref["graph"]), what + ": border chains"
 
 
 @pytest.mark.parametrize("seed", list(range(60)))

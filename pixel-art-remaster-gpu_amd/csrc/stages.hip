@@ -652,21 +652,32 @@ __global__ __launch_bounds__(256) void k_bc_heads(const uint4* __restrict__ F, u
                                                   uint32_t E)
 {
     const uint32_t id = blockIdx.x * 256u + threadIdx.x;
-    if (id >= E || ctr[0] != 0) return;
-    const uint4 st = F[id];
-    if ((int)st.x == kNoLink) return;
-    if ((int)st.x != kEnd) {  // not reached the end: the graph is not a permutation of links
+    const int lane = threadIdx.x & 63;
+    const bool ok = id < E && ctr[0] == 0;
+    const uint4 st = ok ? F[id] : make_uint4((uint32_t)kNoLink, 0u, 0u, 0u);
+    const bool link = (int)st.x != kNoLink;
+    if (link && (int)st.x != kEnd) ctr[0] = DPX_EINVAL;  // not at the end: the links are no permutation
+    const bool head = link && (int)st.x == kEnd && (st.w & 2u);
+    // one atomic per wave: the wave's paths take consecutive ranges
+    const uint64_t len = head ? (uint64_t)st.y + 1u : 0u;
+    uint64_t incl = len;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint64_t t = __shfl_up(incl, d);
+        if (lane >= d) incl += t;
+    }
+    const uint64_t total = __shfl(incl, 63);
+    if (total == 0) return;
+    uint64_t base = 0;
+    if (lane == 0) base = total > E ? (uint64_t)E + 1u : (uint64_t)atomicAdd(reinterpret_cast<unsigned*>(&ctr[1]), (unsigned)total);
+    base = __shfl(base, 0);
+    if (!head) return;
+    const uint64_t off = base + incl - len;
+    if (off + len > E) {
         ctr[0] = DPX_EINVAL;
         return;
     }
-    if (!(st.w & 2u)) return;
-    const uint32_t len = st.y + 1u;
-    const uint32_t off = (uint32_t)atomicAdd(&ctr[1], (int)len);
-    if ((uint64_t)off + len > E) {
-        ctr[0] = DPX_EINVAL;
-        return;
-    }
-    info[st.z] = make_uint4(off, len, st.w & 1u, 0u);
+    info[st.z] = make_uint4((uint32_t)off, (uint32_t)len, st.w & 1u, 0u);
 }
 
 // each link writes its node at its place in its path's array; the first
@@ -689,13 +700,17 @@ __global__ __launch_bounds__(256) void k_bc_scatter(const uint8_t* __restrict__ 
 
 // The scan, one wave (wave 0 of the block).  Marks are two bitmaps
 // (listed, discarded) of (N + 31) / 32 words each, in LDS when they fit,
-// else in bits_g.  Its time is the latency of dependent steps, so what does
-// not depend on the marks is fetched ahead: the other three waves read the
-// path arrays and the graph once, so that the scan's reads of them hit the
-// L2 of its XCD; the next two groups' records are loaded while a group is
-// scanned; the first kBcChunk nodes of the next kBcAhead walks are loaded at
-// once (a walk's nodes do not depend on the marks, only its outcome does).
-// A walk's nodes are tested kBcChunk at a time (kBcU per lane).
+// else in bits_g.  What does not depend on the marks is fetched ahead: the
+// other three waves read the path arrays and the graph once, so that the
+// scan's reads of them hit the L2 of its XCD; the next two groups' records
+// are loaded while a group is scanned; the first kBcChunk nodes of the next
+// kBcAhead walks are loaded at once (a walk's nodes do not depend on the
+// marks, only its outcome does).  A walk's nodes are tested kBcChunk at a
+// time (kBcU per lane).  With the loads ahead, memory waits are a few
+// percent of the scan's cycles: its time is one wave's instruction stream
+// (about 300 instructions per walk, with no other wave to hide their
+// latencies), so deeper prefetching, a third bitmap and staging the output
+// in LDS all measured no faster or slower.
 constexpr int kBcU = 4, kBcChunk = 64 * kBcU, kBcAhead = 4;
 
 __device__ inline int lane_of(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
@@ -755,10 +770,10 @@ __global__ __launch_bounds__(256) void k_bc_scan(const uint8_t* __restrict__ g, 
         const uint4 r = r1;
         v1 = v2;
         r1 = r2;
-        if (me + 128 < N) {  // (a record of a node that starts no walk is never used)
-            v2 = g[me + 128];
-            r2 = rec[me + 128];
-        }
+        // (a record of a node that starts no walk is never used; nodes past
+        // the frame start none)
+        v2 = me + 128 < N ? g[me + 128] : 90u;
+        r2 = me + 128 < N ? rec[me + 128] : make_uint4(0u, 0u, 0u, 0u);
         const bool cand = v != 90u && v != 0u;
         uint64_t m = __ballot(cand && !get(listed, me));
         while (m && status == 0) {

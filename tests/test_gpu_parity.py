@@ -432,6 +432,37 @@ def test_border_chains_match_oracle(gpu, D, O):
         assert got == O.border_chains(g), f"frame {kind}/{seed}"
 
 
+def random_symmetric_graph(seed):
+    """A random graph of links in both directions (not one a frame would
+    give): many pinched faces, where a node's own walk and the chain that
+    lists it differ."""
+    rng = np.random.default_rng(100000 + seed)
+    W, H = int(rng.integers(9, 20)), int(rng.integers(8, 16))
+    p = rng.random() * 0.6 + 0.1
+    g = np.zeros((H, W), dtype=np.uint8)
+    dx, dy = [-1, 0, 1, -1, 1, -1, 0, 1], [1, 1, 1, 0, 0, -1, -1, -1]
+    for j in range(H):
+        for i in range(W):
+            for e in (1, 2, 4, 7):
+                ni, nj = i + dx[e], j + dy[e]
+                if 0 <= ni < W and 0 <= nj < H and rng.random() < p:
+                    g[j, i] |= 1 << e
+                    g[nj, ni] |= 1 << (7 - e)
+    return g
+
+
+def test_border_chains_of_random_graphs(gpu, D, O):
+    """Random symmetric graphs, frames of 72..285 nodes (a last group of
+    64 nodes that the frame does not fill; seeds 102, 1166 and 1698 once
+    gave an extra chain from nodes past the frame's end)."""
+    torch = gpu
+    for seed in [102, 1166, 1698] + list(range(60)):
+        g = random_symmetric_graph(seed)
+        print("graph", seed, g.shape, flush=True)
+        got, nn, status = gpu_border_chains(torch, D, g)
+        assert status == 0 and got == O.border_chains(g), f"graph {seed}"
+
+
 def test_border_chains_refuse_invalid_graphs(gpu, D, O):
     """A link that leaves the frame or has no reverse link: status
     DPX_EINVAL, no chains."""

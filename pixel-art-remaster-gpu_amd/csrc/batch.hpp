@@ -376,22 +376,32 @@ __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ 
     };
     auto tj = [&](int x4, int y4) -> bool { return tjunction(tv, x, y, x4, y4); };
     uint32_t* P = s_poly + t;  // this lane's column
-    int m = 0;
-    subdivide_cell_emit(HullX{ h }, HullY{ h }, n, node, nbr, tj, [&](int px, int py) {
-        P[256 * m] = ((uint32_t)px & 0xFFFFu) | ((uint32_t)py << 16);
-        m++;
-    });
     auto px_of = [](uint32_t p) { return (int)(int16_t)(p & 0xFFFFu); };
     auto py_of = [](uint32_t p) { return (int)p >> 16; };
     // Ear clipping (process, triangulate_functions.cu:108-217) as ear_clip
     // does it: vertices in counter-clockwise order (reversed when the
     // signed area is not positive), first ear tried at (m-1, 0, 1), an ear
     // needs a positive cross and no other vertex inside or on it, a failure
-    // counter of 2 nv.  Each ear's coverage rows are accumulated as it is cut.
+    // counter of 2 nv.  The coverage is the union of the ears' rows.  Pass 0
+    // takes it from the polygon's edges (poly_rows_xor: exact when the
+    // clipping cuts the whole polygon and the winding sum certifies it) and
+    // clips only to learn that; a lane where either fails (none in the
+    // synthetic workloads) takes pass 1, which subdivides again and
+    // accumulates each ear's rows as it is cut.
     uint32_t cov[MW];
+    bool need = true;
+#pragma clang loop unroll(disable)
+    for (int pass = 0; pass < 2; pass++) {
+        if (!need) continue;
+        int m = 0;
+        subdivide_cell_emit(HullX{ h }, HullY{ h }, n, node, nbr, tj, [&](int px, int py) {
+            P[256 * m] = ((uint32_t)px & 0xFFFFu) | ((uint32_t)py << 16);
+            m++;
+        });
 #pragma unroll
-    for (int r = 0; r < MW; r++) cov[r] = 0;
-    if (m >= 3) {
+        for (int r = 0; r < MW; r++) cov[r] = 0;
+        need = false;
+        if (m < 3) continue;
         int a2 = 0;
         for (int p = m - 1, q = 0; q < m; p = q++) {
             const uint32_t A = P[256 * p], B = P[256 * q];
@@ -403,6 +413,8 @@ __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ 
                 P[256 * lo] = P[256 * hi];
                 P[256 * hi] = tmp;
             }
+        int wsum = 0;
+        if (pass == 0) wsum = poly_rows_xor<MW>(P, 256, m, 2 * S, KMIN, cov);
         int nv = m, count = 2 * nv;
         for (int v = nv - 1; nv > 2;) {
             if (0 >= (count--)) break;  // the reference gives up; the triangles cut so far stand
@@ -428,11 +440,19 @@ __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ 
             if (ear) {
                 // coverage of triangle (A, B, C) in units of 1/(128 S) of a
                 // cell; it is counter-clockwise (positive cross)
-                tri_rows_or<MW>(ax * 2 * S, ay * 2 * S, bx * 2 * S, by * 2 * S, cx_ * 2 * S, cy_ * 2 * S, KMIN, cov);
+                if (pass == 1)
+                    tri_rows_or<MW>(ax * 2 * S, ay * 2 * S, bx * 2 * S, by * 2 * S, cx_ * 2 * S, cy_ * 2 * S, KMIN,
+                                    cov);
                 for (int s2 = v, t2 = v + 1; t2 < nv; s2++, t2++) P[256 * s2] = P[256 * t2];
                 nv--;
                 count = 2 * nv;
             }
+        }
+        if (pass == 0) {
+            int bits = 0;
+#pragma unroll
+            for (int r = 0; r < MW; r++) bits += __builtin_popcount(cov[r]);
+            need = nv > 2 || wsum != bits;
         }
     }
     uint32_t* dst = rows + it.rows_off / 4 + (size_t)((uint32_t)y * (uint32_t)it.W + (uint32_t)x) * RW;

@@ -514,4 +514,60 @@ __host__ __device__ inline void tri_rows_or(int ax, int ay, int bx, int by, int 
     }
 }
 
+// The coverage rows of a counter-clockwise polygon that ear clipping has
+// cut completely into triangles, in one pass over its edges instead of one
+// over each triangle.  Under edge_in's rule a sample is covered by a
+// triangle iff the sample moved by (dx, dy), 0 < dy << dx infinitesimal, is
+// inside it; the number of the cut (counter-clockwise) triangles covering a
+// sample is then the polygon's winding number w >= 0 around the moved
+// sample (the triangles' boundaries add up to the polygon's: each cut edge
+// enters twice, opposite ways), and w = sum, over the edges that cross the
+// moved row (min(py, qy) <= sy < max(py, qy)) with the sample left of the
+// crossing, of +1 (upward edge) or -1 (downward).  The columns left of the
+// crossing: an upward edge's covered side (tri_rows_or's hi bound), a
+// downward edge's uncovered side (below its lo bound).  The function
+// returns in cov the parity of w per sample (the exclusive or of those
+// column masks) and in the result the sum of w over the window's samples
+// (the masks' signed bit counts).  The union of the triangles (w >= 1) is
+// the parity exactly when every w is 0 or 1, i.e. when that sum equals the
+// number of set bits of cov: the caller's check.
+// P: vertex k at P[stride * k], x | y << 16 in 1/64 units of a cell;
+// coordinates are scaled by 2 S as for tri_rows_or.
+template <int MW>
+__host__ __device__ inline int poly_rows_xor(const uint32_t* P, int stride, int m, int S2, int kmin, uint32_t* cov)
+{
+    const int sy0 = 128 * kmin + 64;
+    int wsum = 0;
+    for (int k = 0; k < m; k++) {
+        const uint32_t A = P[stride * k], B = P[stride * (k + 1 == m ? 0 : k + 1)];
+        const int px = (int)(int16_t)(A & 0xFFFFu) * S2, py = ((int)A >> 16) * S2;
+        const int qx = (int)(int16_t)(B & 0xFFFFu) * S2, qy = ((int)B >> 16) * S2;
+        const int dx = qx - px, dy = qy - py;
+        if (dy == 0) continue;
+        const int ylo = min(py, qy), yhi = max(py, qy);
+        const int D = 128 * (dy < 0 ? -dy : dy);
+        // column bound at row 0 (as tri_rows_or): N = dx (sy0 - py) + dy (px - sy0)
+        const int N0 = dx * (sy0 - py) + dy * (px - sy0) - (dy > 0 ? 1 : 0);
+        int q = floor_div(N0, D), rem = N0 - q * D;
+        const int qs = floor_div(128 * dx, D), rs = 128 * dx - qs * D;
+#pragma unroll
+        for (int r = 0; r < MW; r++) {
+            const int sy = sy0 + 128 * r;
+            const int b = dy > 0 ? q : -q - 1;  // last column left of the crossing
+            const uint32_t msk = b < 0 ? 0u : (b >= MW - 1 ? (MW == 32 ? ~0u : (1u << MW) - 1u) : (2u << b) - 1u);
+            if (sy >= ylo && sy < yhi) {
+                cov[r] ^= msk;
+                wsum += dy > 0 ? __builtin_popcount(msk) : -__builtin_popcount(msk);
+            }
+            q += qs;
+            rem += rs;
+            if (rem >= D) {
+                rem -= D;
+                q++;
+            }
+        }
+    }
+    return wsum;
+}
+
 }  // namespace dpx

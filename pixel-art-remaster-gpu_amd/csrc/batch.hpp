@@ -190,12 +190,34 @@ __device__ inline void tile_snapshot_load(const DevItem& it, const uint8_t* __re
 // positions per thread, into s_g32 (resolved rows y0 - 1 .. y0 + RTY), and
 // the tile's own resolved bytes into gfin when it is given.  s_p32 must
 // have been cleared and *s_nblk zeroed before the barrier.
+//
+// The curve walks of the decisions (up to 31 steps) leave the small window
+// on line-heavy frames, and each step outside it is a dependent HBM read.
+// With WALK, a tile that has crossed blocks first loads a larger walk window
+// of the snapshot into s_walk (an LDS buffer of at least kWalkBytes that the
+// kernel uses only later): kWalkEX more bytes on each side and kWalkEY more
+// rows above and below.  It costs every such tile 6-8 KB more reads, which
+// a full batch pays in HBM time (15% of the 256x224 batch), but a batch of a
+// few frames is bound by the latency of its slowest tile's walks instead:
+// the host picks WALK for small launches.
+constexpr int kWalkEX = 32, kWalkEY = 24;
 template <int TX>
+struct WalkWindow {
+    static constexpr int B = TileShape<TX>::DB + 2 * kWalkEX;    // bytes per row (a multiple of 4)
+    static constexpr int R = TileShape<TX>::AY + 2 * kWalkEY;    // rows
+    static constexpr int BYTES = B * R;
+};
+constexpr int kWalkBytes = 8192;  // room the callers provide (the largest shape's window fits)
+static_assert(WalkWindow<16>::BYTES <= kWalkBytes && WalkWindow<32>::BYTES <= kWalkBytes &&
+                  WalkWindow<64>::BYTES <= kWalkBytes, "walk windows fit their buffer");
+
+template <int TX, bool WALK>
 __device__ inline void tile_resolve(const DevItem& it, const uint8_t* __restrict__ aux, uint8_t* __restrict__ gfin,
                                     int x0, int y0, int ay0, const uint32_t* s_aux32, uint32_t* s_p32,
-                                    uint16_t* s_blk, int* s_nblk, uint32_t* s_g32, int t)
+                                    uint16_t* s_blk, int* s_nblk, uint32_t* s_g32, uint32_t* s_walk, int t)
 {
     DPX_TILE_NAMES(TX);
+    using WW_ = WalkWindow<TX>;
     for (int k = t; k < (GY + 1) * RDW; k += 256) {
         const int r = k / RDW, w = k % RDW + 1;  // word w of snapshot row r: origins of blocks on rows r, r + 1
         const uint32_t* a = s_aux32 + r * RWW + w;
@@ -209,14 +231,45 @@ __device__ inline void tile_resolve(const DevItem& it, const uint8_t* __restrict
     }
     __syncthreads();
     const int nblk = *s_nblk;
+    const bool walk_win = WALK && nblk > 0;  // uniform over the block
+    const int ex0 = x0 - 4 - kWalkEX, ey0 = ay0 - kWalkEY;  // the walk window's first byte and row
+    if (walk_win) {
+        if (it.graph_w4) {  // x0 and kWalkEX are multiples of 4: whole words inside or outside the frame
+            for (int k = t; k < WW_::R * (WW_::B / 4); k += 256) {
+                const int r = k / (WW_::B / 4), w = k % (WW_::B / 4);
+                const int x = ex0 + 4 * w, y = ey0 + r;
+                uint32_t v = 0;
+                if ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
+                    v = *reinterpret_cast<const uint32_t*>(aux + (uint32_t)(y * it.W + x));
+                s_walk[k] = v;
+            }
+        } else {
+            uint8_t* s8 = reinterpret_cast<uint8_t*>(s_walk);
+            for (int k = t; k < WW_::BYTES; k += 256) {
+                const int x = ex0 + k % WW_::B, y = ey0 + k / WW_::B;
+                s8[k] = ((unsigned)x < (unsigned)it.W && (unsigned)y < (unsigned)it.H)
+                            ? aux[(uint32_t)(y * it.W + x)] : 0;
+            }
+        }
+        __syncthreads();
+    }
     {
         // (the window is read by words, so that the compiler keeps the LDS
         // and HBM reads apart instead of merging them into one generic load)
         auto get = [&](int gx, int gy) -> uint32_t {
-            int wx = gx - (x0 - 4), wy = gy - ay0;
-            bool win = (unsigned)wx < (unsigned)RDB && (unsigned)wy < (unsigned)AY;
-            int c = win ? wy * RWB + wx + 4 : 0;
-            uint32_t v = (s_aux32[c >> 2] >> (8 * (c & 3))) & 0xFFu;
+            bool win;
+            uint32_t v;
+            if (walk_win) {
+                const int wx = gx - ex0, wy = gy - ey0;
+                win = (unsigned)wx < (unsigned)WW_::B && (unsigned)wy < (unsigned)WW_::R;
+                const int c = win ? wy * WW_::B + wx : 0;
+                v = (s_walk[c >> 2] >> (8 * (c & 3))) & 0xFFu;
+            } else {
+                const int wx = gx - (x0 - 4), wy = gy - ay0;
+                win = (unsigned)wx < (unsigned)RDB && (unsigned)wy < (unsigned)AY;
+                const int c = win ? wy * RWB + wx + 4 : 0;
+                v = (s_aux32[c >> 2] >> (8 * (c & 3))) & 0xFFu;
+            }
             if (!win) v = aux[(uint32_t)(gy * it.W + gx)];
             return v;
         };
@@ -295,7 +348,7 @@ __device__ inline void tile_resolved_load(const DevItem& it, const uint8_t* __re
 // so that every lane of a wave works on one; each lane's polygon lives in
 // LDS (a column of s_poly), where ear clipping scans and shifts it.
 // ---------------------------------------------------------------------------
-template <int S, int TX>
+template <int S, int TX, bool WALK>
 __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
                                                  int grid3d, const uint8_t* __restrict__ in,
                                                  const uint8_t* __restrict__ gaux, uint8_t* __restrict__ gfin,
@@ -331,7 +384,8 @@ __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ 
     if (t < (GY + 1) * RWW) s_p32[t] = 0;
     tile_snapshot_load<TX>(it, aux, x0, ay0, s_aux32, t);
     __syncthreads();
-    tile_resolve<TX>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, t);
+    static_assert(16 * 256 * 4 >= kWalkBytes, "the polygon buffer holds the walk window before it holds polygons");
+    tile_resolve<TX, WALK>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, s_poly, t);
     __syncthreads();
     // hulls of the own cells and of their neighbours: cell (x0 - 1 + lx,
     // y0 - 1 + ly) at resolved row ly, byte lx + 7
@@ -465,7 +519,7 @@ __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ 
 // With SUB the resolved graph comes from gfin and the coverage rows of the
 // smoothed cells from `cell_rows` (kf_smooth ran before); without, the kernel
 // resolves the crossings itself from the snapshot.
-template <int S, bool SUB, int TX>
+template <int S, bool SUB, int TX, bool WALK>
 __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
                                                  int grid3d,
                                                  const uint8_t* __restrict__ in, const uint8_t* __restrict__ gaux,
@@ -548,7 +602,8 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     }
     __syncthreads();
     if constexpr (!SUB) {
-        tile_resolve<TX>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, t);
+        static_assert(PX_WORDS * 4 >= kWalkBytes, "the pixel buffer holds the walk window before it holds pixels");
+        tile_resolve<TX, WALK>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, s_px, t);
         __syncthreads();
     }
     // candidate cells: key and coverage summary.  Candidate (lx, ly) is the
@@ -955,20 +1010,22 @@ using SmoothFn = void (*)(const DevItem*, const uint2*, int, const uint8_t*, con
 
 // The raster and smoothing kernels of scale S (1..15), per tile shape (one
 // translation unit each: raster_tiles.hip).
-RasterFn pick_raster_t16(int S, bool sub);
-RasterFn pick_raster_t32(int S, bool sub);
-RasterFn pick_raster_t64(int S, bool sub);
-SmoothFn pick_smooth_t16(int S);
-SmoothFn pick_smooth_t32(int S);
-SmoothFn pick_smooth_t64(int S);
+// walk: the variants that load the walk window (tile_resolve's WALK)
+RasterFn pick_raster_t16(int S, bool sub, bool walk);
+RasterFn pick_raster_t32(int S, bool sub, bool walk);
+RasterFn pick_raster_t64(int S, bool sub, bool walk);
+SmoothFn pick_smooth_t16(int S, bool walk);
+SmoothFn pick_smooth_t32(int S, bool walk);
+SmoothFn pick_smooth_t64(int S, bool walk);
 
-inline RasterFn pick_raster(int S, bool sub, int tx)
+inline RasterFn pick_raster(int S, bool sub, int tx, bool walk = false)
 {
-    return tx == 16 ? pick_raster_t16(S, sub) : tx == 64 ? pick_raster_t64(S, sub) : pick_raster_t32(S, sub);
+    return tx == 16 ? pick_raster_t16(S, sub, walk) : tx == 64 ? pick_raster_t64(S, sub, walk)
+                                                              : pick_raster_t32(S, sub, walk);
 }
-inline SmoothFn pick_smooth(int S, int tx)
+inline SmoothFn pick_smooth(int S, int tx, bool walk = false)
 {
-    return tx == 16 ? pick_smooth_t16(S) : tx == 64 ? pick_smooth_t64(S) : pick_smooth_t32(S);
+    return tx == 16 ? pick_smooth_t16(S, walk) : tx == 64 ? pick_smooth_t64(S, walk) : pick_smooth_t32(S, walk);
 }
 
 }  // namespace dpx

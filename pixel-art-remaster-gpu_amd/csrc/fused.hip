@@ -291,7 +291,14 @@ struct PlanGroup {
     dim3 grid;
     int grid3d;
     size_t off_items, off_map;  // in the descriptor buffer
+    bool walk;  // the crossing walks read a larger LDS window (small launches)
 };
+// Launches of at most this many blocks resolve crossings with the larger
+// walk window (tile_resolve's WALK): a grid of about one block per CU runs
+// as long as its slowest tile, whose curve walks step through HBM beyond the
+// small window one dependent read at a time; larger grids hide that latency
+// behind other blocks and would pay the window's extra reads in HBM time.
+constexpr uint64_t kWalkWindowBlocks = 2048;
 struct BatchPlan {
     bool valid = false;
     std::vector<dpx_item> items;
@@ -723,7 +730,8 @@ static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* it
         const Group& grp = kv.second;
         size_t a = put(grp.dev.data(), grp.dev.size() * sizeof(DevItem));
         size_t b = put(grp.map.data(), grp.map.size() * sizeof(uint2));
-        plan.groups.push_back(PlanGroup{ grp.scale, grp.sub, grp.tx, grp.grid, grp.grid3d, a, b });
+        const bool walk = (uint64_t)grp.grid.x * grp.grid.y * grp.grid.z <= kWalkWindowBlocks;
+        plan.groups.push_back(PlanGroup{ grp.scale, grp.sub, grp.tx, grp.grid, grp.grid3d, a, b, walk });
     }
     plan.items.assign(items, items + n_items);
     plan.out_mod = reinterpret_cast<uintptr_t>(d_out) % 16;
@@ -780,12 +788,12 @@ static int launch_plan(dpx_context* ctx, const BatchPlan& plan, const LaunchArgs
     hipLaunchKernelGGL(kf_graph, plan.ggrid, dim3(256), 0, stream, d_items, d_gmap, plan.g3d, d_in, la.gaux);
     if (ev_mid) DPX_HIP(hipEventRecord(ev_mid, stream));
     for (const PlanGroup& pg : plan.groups) {
-        RasterFn fn = pick_raster(pg.scale, pg.sub, pg.tx);
+        RasterFn fn = pick_raster(pg.scale, pg.sub, pg.tx, pg.walk);
         DPX_CHECK(fn, DPX_EINVAL, "no raster kernel for scale %d", pg.scale);
         const DevItem* d_gitems = reinterpret_cast<const DevItem*>(la.desc + pg.off_items);
         const uint2* d_rmap = reinterpret_cast<const uint2*>(la.desc + pg.off_map);
         if (pg.sub) {
-            SmoothFn sf = pick_smooth(pg.scale, pg.tx);
+            SmoothFn sf = pick_smooth(pg.scale, pg.tx, pg.walk);
             DPX_CHECK(sf && la.gfin && la.rows, DPX_EINVAL, "no smoothing buffers for scale %d", pg.scale);
             hipLaunchKernelGGL(sf, pg.grid, dim3(256), 0, stream, d_gitems, d_rmap, pg.grid3d, d_in, la.gaux, la.gfin,
                                ctx->d_hull, la.rows);
@@ -916,13 +924,13 @@ hipKernelNodeParams node_params(dpx_plan* p, int kind, int group, const uint8_t*
             a.hull = p->ctx->d_hull;
             void* v[8] = { &a.items, &a.map, &a.g3d, &a.in, &a.gaux_c, &a.gfin, &a.hull, &a.rows };
             memcpy(a.ptrs, v, sizeof(v));
-            np.func = reinterpret_cast<void*>(pick_smooth(pg.scale, pg.tx));
+            np.func = reinterpret_cast<void*>(pick_smooth(pg.scale, pg.tx, pg.walk));
         } else {
             a.mask = p->ctx->d_mask[pg.scale];
             a.info = p->ctx->d_info[pg.scale];
             void* v[10] = { &a.items, &a.map, &a.g3d, &a.in, &a.gaux_c, &a.gfin, &a.rows_c, &a.mask, &a.info, &a.out };
             memcpy(a.ptrs, v, sizeof(v));
-            np.func = reinterpret_cast<void*>(pick_raster(pg.scale, pg.sub, pg.tx));
+            np.func = reinterpret_cast<void*>(pick_raster(pg.scale, pg.sub, pg.tx, pg.walk));
         }
         np.gridDim = pg.grid;
     }

@@ -194,22 +194,21 @@ __device__ inline void tile_snapshot_load(const DevItem& it, const uint8_t* __re
 // The curve walks of the decisions (up to 31 steps) leave the small window
 // on line-heavy frames, and each step outside it is a dependent HBM read.
 // With WALK, a tile that has crossed blocks first loads a larger walk window
-// of the snapshot into s_walk (an LDS buffer of at least kWalkBytes that the
-// kernel uses only later): kWalkEX more bytes on each side and kWalkEY more
-// rows above and below.  It costs every such tile 6-8 KB more reads, which
+// of the snapshot into s_walk (an LDS buffer of WalkWindow<TX>::BYTES that
+// the kernel uses only later): kWalkEX more bytes on each side and kWalkEY
+// more rows above and below, so that no walk (at most 31 steps from a node
+// next to a block of the small window) leaves it.  It costs every such tile
+// 7-10 KB more reads, which
 // a full batch pays in HBM time (15% of the 256x224 batch), but a batch of a
 // few frames is bound by the latency of its slowest tile's walks instead:
 // the host picks WALK for small launches.
-constexpr int kWalkEX = 32, kWalkEY = 24;
+constexpr int kWalkEX = 32, kWalkEY = 32;
 template <int TX>
 struct WalkWindow {
     static constexpr int B = TileShape<TX>::DB + 2 * kWalkEX;    // bytes per row (a multiple of 4)
     static constexpr int R = TileShape<TX>::AY + 2 * kWalkEY;    // rows
     static constexpr int BYTES = B * R;
 };
-constexpr int kWalkBytes = 8192;  // room the callers provide (the largest shape's window fits)
-static_assert(WalkWindow<16>::BYTES <= kWalkBytes && WalkWindow<32>::BYTES <= kWalkBytes &&
-                  WalkWindow<64>::BYTES <= kWalkBytes, "walk windows fit their buffer");
 
 template <int TX, bool WALK>
 __device__ inline void tile_resolve(const DevItem& it, const uint8_t* __restrict__ aux, uint8_t* __restrict__ gfin,
@@ -384,7 +383,7 @@ __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ 
     if (t < (GY + 1) * RWW) s_p32[t] = 0;
     tile_snapshot_load<TX>(it, aux, x0, ay0, s_aux32, t);
     __syncthreads();
-    static_assert(16 * 256 * 4 >= kWalkBytes, "the polygon buffer holds the walk window before it holds polygons");
+    static_assert(16 * 256 * 4 >= WalkWindow<TX>::BYTES, "the polygon buffer holds the walk window first");
     tile_resolve<TX, WALK>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, s_poly, t);
     __syncthreads();
     // hulls of the own cells and of their neighbours: cell (x0 - 1 + lx,
@@ -602,8 +601,8 @@ __global__ __launch_bounds__(256) void kf_raster(const DevItem* __restrict__ ite
     }
     __syncthreads();
     if constexpr (!SUB) {
-        static_assert(PX_WORDS * 4 >= kWalkBytes, "the pixel buffer holds the walk window before it holds pixels");
-        tile_resolve<TX, WALK>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, s_px, t);
+        __shared__ uint32_t s_walk[WALK ? WalkWindow<TX>::BYTES / 4 : 1];
+        tile_resolve<TX, WALK>(it, aux, gfin, x0, y0, ay0, s_aux32, s_p32, s_blk, &s_nblk, s_g32, s_walk, t);
         __syncthreads();
     }
     // candidate cells: key and coverage summary.  Candidate (lx, ly) is the

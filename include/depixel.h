@@ -238,6 +238,19 @@ int dpx_batch_run(dpx_context* ctx, const uint8_t* d_in, uint8_t* d_out, const d
 #define DPX_CLASS_LARGE  2  /* larger                                  */
 int dpx_context_set_tile(dpx_context* ctx, int size_class, int scale, int subdivide, int tile_w);
 int dpx_context_get_tile(dpx_context* ctx, int size_class, int scale, int subdivide, int* tile_w);
+/* Walk window per (size class, scale, subdivision), the crossing decisions'
+ * halo: with it a tile loads 32 more bytes and rows of the graph snapshot on
+ * each side into LDS before deciding, so that no curve walk reads HBM one
+ * step at a time; without it walks that leave the tile's small window do.
+ * DPX_WALK_AUTO (the default): for launches of at most 2048 blocks only,
+ * where a tile's walk latency is the launch's; DPX_WALK_ALWAYS: every
+ * launch (picked by the autotuner where it is faster); DPX_WALK_NEVER.
+ * The results are the same in every mode. */
+#define DPX_WALK_AUTO   0
+#define DPX_WALK_ALWAYS 1
+#define DPX_WALK_NEVER  2
+int dpx_context_set_walk(dpx_context* ctx, int size_class, int scale, int subdivide, int walk);
+int dpx_context_get_walk(dpx_context* ctx, int size_class, int scale, int subdivide, int* walk);
 /* Whether the batches that follow also write the resolved graph of every
  * item (1 byte per pixel, the reference's graph_h): off by default, since
  * the images are the product and the graph writes cost about 3.5% of the

@@ -1,13 +1,16 @@
-"""Autotuner of the batched path's raster tile shape.
+"""Autotuner of the batched path's launch shape: raster tile shape and the
+crossing decisions' halo.
 
 For each workload class (sprites, console-size frames, large frames) at the
 scales it is measured on, with and without subdivision, every tile shape
-(16x16, 32x8 and 64x4 cells per block) is timed on the device with the
-context's own kernel timing (HIP events on the launch stream), the outputs
-of the three shapes are checked to be identical, and the fastest shape per
-(size class, scale, subdivision) is written to tuning.json next to this
-file, which every dpx_context reads when it is created.  A shape replaces
-the default (32x8) only when it is faster by more than `--margin`.
+(16x16, 32x8 and 64x4 cells per block) with and without the walk window (a
+32-pixel halo of the graph snapshot in LDS, so that no curve walk reads
+HBM) is timed on the device with the context's own kernel timing (HIP
+events on the launch stream), the outputs of all six are checked to be
+identical, and the fastest per (size class, scale, subdivision) is written
+to tuning.json next to this file, which every dpx_context reads when it is
+created.  A choice replaces the default (32x8, no walk window for launches
+of more than 2048 blocks) only when it is faster by more than `--margin`.
 
     python pixel-art-remaster-gpu_amd/autotune.py [--out tuning.json] [--runs 10]
 
@@ -26,6 +29,14 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 CLASS_NAMES = {0: "sprite", 1: "frame", 2: "large"}
+# (tile width, walk window); the default is 32x8 without the window
+MODES = [(tx, walk) for tx in (16, 32, 64) for walk in (False, True)]
+DEFAULT = (32, False)
+
+
+def mode_name(m):
+    tx, walk = m
+    return f"{tx}x{256 // tx}" + ("+walk" if walk else "")
 
 
 def workloads(D):
@@ -82,9 +93,10 @@ def main():
         d_out = torch.empty(out_off, dtype=torch.uint8, device=dev)
         arr = D.items_array(items)
         times, digests = {}, {}
-        for tx in D.TILE_WIDTHS:
+        for tx, walk in MODES:
             ctx = D.Context(0)
             ctx.set_tile(cls, scale, bool(sub), tx)
+            ctx.set_walk(cls, scale, bool(sub), D.WALK_ALWAYS if walk else D.WALK_NEVER)
             for _ in range(3):
                 ctx.run(d_in, d_out, arr, stream)
             torch.cuda.synchronize()
@@ -93,39 +105,40 @@ def main():
             for _ in range(args.runs):
                 ctx.run(d_in, d_out, arr, stream)
             t = ctx.timing_read()
-            times[tx] = t.raster_ms / max(1, t.batches)
-            digests[tx] = hashlib.sha1(d_out.cpu().numpy().tobytes()).hexdigest()
+            times[(tx, walk)] = t.raster_ms / max(1, t.batches)
+            digests[(tx, walk)] = hashlib.sha1(d_out.cpu().numpy().tobytes()).hexdigest()
             ctx.close()
-        assert len(set(digests.values())) == 1, f"{label}: the tile shapes disagree: {digests}"
+        assert len(set(digests.values())) == 1, f"{label}: the launch shapes disagree: {digests}"
         best = min(times, key=times.get)
-        if times[best] > times[32] * (1 - args.margin):
-            best = 32
+        if times[best] > times[DEFAULT] * (1 - args.margin):
+            best = DEFAULT
         key = (cls, scale, sub)
         # several workloads of one key: keep the pick of the first (the listed order puts the headline first)
         if key not in picks:
             picks[key] = (best, label, times)
         table.append((label, CLASS_NAMES[cls], scale, sub, times, best))
-        print(f"{label}: " + ", ".join(f"{tx}: {times[tx]:.4f} ms" for tx in D.TILE_WIDTHS) + f" -> {best}",
-              flush=True)
+        print(f"{label}: " + ", ".join(f"{mode_name(m)}: {times[m]:.4f} ms" for m in MODES) +
+              f" -> {mode_name(best)}", flush=True)
     doc = {
         "device": arch,
         "generated_by": "pixel-art-remaster-gpu_amd/autotune.py",
         "generated_at": time.strftime("%Y-%m-%d %H:%M:%S"),
         "metric": "smoothing + raster kernel time per batch (ms), HIP events on the launch stream",
         "picks": [
-            {"class": CLASS_NAMES[c], "scale": s, "subdivide": sb, "tile": best, "workload": label,
-             "ms": {str(k): round(v, 5) for k, v in times.items()},
-             "gain_vs_default": round(1 - times[best] / times[32], 4)}
+            {"class": CLASS_NAMES[c], "scale": s, "subdivide": sb, "tile": best[0], "walk": int(best[1]),
+             "workload": label, "ms": {mode_name(k): round(v, 5) for k, v in times.items()},
+             "gain_vs_default": round(1 - times[best] / times[DEFAULT], 4)}
             for (c, s, sb), (best, label, times) in sorted(picks.items())
         ],
     }
     with open(args.out, "w") as f:
         json.dump(doc, f, indent=1)
-    print("| workload | class | scale | subdivide | 16x16 ms | 32x8 ms | 64x4 ms | pick | gain vs 32x8 |")
-    print("|---|---|---|---|---|---|---|---|---|")
+    print("| workload | class | scale | subdivide | " + " | ".join(f"{mode_name(m)} ms" for m in MODES) +
+          " | pick | gain vs 32x8 |")
+    print("|---|---|---|---|" + "---|" * len(MODES) + "---|---|")
     for label, c, s, sb, times, best in table:
-        print(f"| {label} | {c} | {s} | {sb} | {times[16]:.4f} | {times[32]:.4f} | {times[64]:.4f} | "
-              f"{best} | {1 - times[best] / times[32]:.1%} |")
+        print(f"| {label} | {c} | {s} | {sb} | " + " | ".join(f"{times[m]:.4f}" for m in MODES) +
+              f" | {mode_name(best)} | {1 - times[best] / times[DEFAULT]:.1%} |")
     print(f"wrote {args.out}")
 
 

@@ -331,6 +331,9 @@ struct dpx_context {
     bool last_kept = false;   // the last batch wrote them
     // raster tile width per (size class, scale, subdivision); 0: the default
     int8_t tile_pick[3][dpx::kMaxScale + 1][2] = {};
+    // walk window per (size class, scale, subdivision): DPX_WALK_AUTO (small
+    // launches only), DPX_WALK_ALWAYS or DPX_WALK_NEVER
+    int8_t walk_pick[3][dpx::kMaxScale + 1][2] = {};
     std::mutex mu;
     // kernel timing: 3 events per recorded batch (start, after graph, end)
     bool timing = false;
@@ -357,9 +360,15 @@ int item_tile(const dpx_context* ctx, const dpx_item& a)
     return t ? t : kDefaultTileX;
 }
 
+int item_walk(const dpx_context* ctx, const dpx_item& a)
+{
+    return ctx->walk_pick[size_class(a.width, a.height)][a.scale][a.subdivide ? 1 : 0];
+}
+
 // The tile picks of a tuning file: a JSON object whose "picks" array holds
 // objects {"class": "sprite" | "frame" | "large", "scale": 1..15,
-// "subdivide": 0 | 1, "tile": 16 | 32 | 64} (other keys are ignored).
+// "subdivide": 0 | 1, "tile": 16 | 32 | 64, "walk": 0 | 1} (other keys are
+// ignored; "walk": 1 picks the walk window for every launch of the key).
 // Returns the number of picks applied.
 int load_tuning(dpx_context* ctx, const char* path)
 {
@@ -385,14 +394,16 @@ int load_tuning(dpx_context* ctx, const char* path)
             *v = atoi(obj.c_str() + c + 1);
             return true;
         };
-        int cls = -1, scale = 0, sub = 0, tile = 0;
+        int cls = -1, scale = 0, sub = 0, tile = 0, walk = 0;
         if (obj.find("\"sprite\"") != std::string::npos) cls = DPX_CLASS_SPRITE;
         else if (obj.find("\"frame\"") != std::string::npos) cls = DPX_CLASS_FRAME;
         else if (obj.find("\"large\"") != std::string::npos) cls = DPX_CLASS_LARGE;
         if (cls < 0 || !num("\"scale\"", &scale) || !num("\"tile\"", &tile)) continue;
         num("\"subdivide\"", &sub);
+        num("\"walk\"", &walk);
         if (scale < 1 || scale > kMaxScale || (tile != 16 && tile != 32 && tile != 64)) continue;
         ctx->tile_pick[cls][scale][sub ? 1 : 0] = (int8_t)tile;
+        ctx->walk_pick[cls][scale][sub ? 1 : 0] = (int8_t)(walk == 1 ? DPX_WALK_ALWAYS : DPX_WALK_AUTO);
         applied++;
     }
     return applied;
@@ -434,6 +445,7 @@ struct Group {
     int scale;
     bool sub;
     int tx = kDefaultTileX;
+    int walk = DPX_WALK_AUTO;
     std::vector<DevItem> dev;  // the group's items, in launch order
     std::vector<uint2> map;    // tile map of a 1-D launch
     dim3 grid;
@@ -616,6 +628,29 @@ extern "C" int dpx_context_get_tile(dpx_context* ctx, int size_class, int scale,
     return DPX_OK;
 }
 
+extern "C" int dpx_context_set_walk(dpx_context* ctx, int size_class, int scale, int subdivide, int walk)
+{
+    DPX_CHECK(ctx, DPX_EINVAL, "null context");
+    DPX_CHECK(size_class >= 0 && size_class <= 2, DPX_EINVAL, "size class %d", size_class);
+    DPX_CHECK(scale >= 1 && scale <= kMaxScale, DPX_EINVAL, "scale %d outside 1..%d", scale, kMaxScale);
+    DPX_CHECK(walk == DPX_WALK_AUTO || walk == DPX_WALK_ALWAYS || walk == DPX_WALK_NEVER, DPX_EINVAL,
+              "walk window mode %d", walk);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    ctx->walk_pick[size_class][scale][subdivide ? 1 : 0] = (int8_t)walk;
+    ctx->plan.valid = false;  // the next batch plans its groups again
+    return DPX_OK;
+}
+
+extern "C" int dpx_context_get_walk(dpx_context* ctx, int size_class, int scale, int subdivide, int* walk)
+{
+    DPX_CHECK(ctx && walk, DPX_EINVAL, "bad arguments");
+    DPX_CHECK(size_class >= 0 && size_class <= 2, DPX_EINVAL, "size class %d", size_class);
+    DPX_CHECK(scale >= 1 && scale <= kMaxScale, DPX_EINVAL, "scale %d outside 1..%d", scale, kMaxScale);
+    std::lock_guard<std::mutex> lock(ctx->mu);
+    *walk = ctx->walk_pick[size_class][scale][subdivide ? 1 : 0];
+    return DPX_OK;
+}
+
 extern "C" int dpx_context_keep_graph(dpx_context* ctx, int enable)
 {
     DPX_CHECK(ctx, DPX_EINVAL, "null context");
@@ -695,10 +730,11 @@ static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* it
             d.rows_off = roff;
             roff += align_up((size_t)a.width * a.height * rows_words(a.scale) * 4, 256);
         }
-        const int tx = item_tile(ctx, a);
-        const int gkey = tx << 8 | a.scale << 1 | d.subdivide;
+        const int tx = item_tile(ctx, a), wk = item_walk(ctx, a);
+        const int gkey = wk << 12 | tx << 8 | a.scale << 1 | d.subdivide;
         Group& grp = groups[gkey];
         grp.tx = tx;
+        grp.walk = wk;
         grp.scale = a.scale;
         grp.sub = d.subdivide;
         grp.dev.push_back(d);
@@ -730,7 +766,9 @@ static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* it
         const Group& grp = kv.second;
         size_t a = put(grp.dev.data(), grp.dev.size() * sizeof(DevItem));
         size_t b = put(grp.map.data(), grp.map.size() * sizeof(uint2));
-        const bool walk = (uint64_t)grp.grid.x * grp.grid.y * grp.grid.z <= kWalkWindowBlocks;
+        const bool walk = grp.walk == DPX_WALK_ALWAYS ||
+                          (grp.walk == DPX_WALK_AUTO &&
+                           (uint64_t)grp.grid.x * grp.grid.y * grp.grid.z <= kWalkWindowBlocks);
         plan.groups.push_back(PlanGroup{ grp.scale, grp.sub, grp.tx, grp.grid, grp.grid3d, a, b, walk });
     }
     plan.items.assign(items, items + n_items);

@@ -20,6 +20,7 @@ HEADER_PATH = os.path.join(os.path.dirname(os.path.dirname(_HERE)), "include", "
 CELL_SIZE = 45
 RGB, TOPDOWN, ALPHA, FLIP_OUT, INDEXED = 1, 2, 4, 8, 16
 CLASS_SPRITE, CLASS_FRAME, CLASS_LARGE = 0, 1, 2
+WALK_AUTO, WALK_ALWAYS, WALK_NEVER = 0, 1, 2
 TILE_WIDTHS = (16, 32, 64)
 VIEW_CELLS, VIEW_QUADS = 0, 1
 VIEW_AA, VIEW_TOPDOWN = 1, 2
@@ -103,6 +104,8 @@ _SIGS = {
     "dpx_context_keep_graph": (c_int, [c_void_p, c_int]),
     "dpx_context_set_tile": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
     "dpx_context_get_tile": (c_int, [c_void_p, c_int, c_int, c_int, POINTER(c_int)]),
+    "dpx_context_set_walk": (c_int, [c_void_p, c_int, c_int, c_int, c_int]),
+    "dpx_context_get_walk": (c_int, [c_void_p, c_int, c_int, c_int, POINTER(c_int)]),
     "dpx_context_reserve": (c_int, [c_void_p, c_int, ctypes.c_int64, c_int]),
     "dpx_batch_timing": (c_int, [c_void_p, c_int]),
     "dpx_plan_create": (c_int, [c_void_p, POINTER(Item), c_int, c_uint64, c_uint32, POINTER(c_void_p)]),
@@ -288,6 +291,15 @@ class Context:
         t = c_int()
         check(lib.dpx_context_get_tile(self.h, size_class, scale, int(subdivide), ctypes.byref(t)))
         return t.value
+
+    def set_walk(self, size_class: int, scale: int, subdivide: bool, walk: int):
+        """Walk window mode (WALK_AUTO, WALK_ALWAYS, WALK_NEVER) per key."""
+        check(lib.dpx_context_set_walk(self.h, size_class, scale, int(subdivide), walk))
+
+    def get_walk(self, size_class: int, scale: int, subdivide: bool) -> int:
+        w = c_int()
+        check(lib.dpx_context_get_walk(self.h, size_class, scale, int(subdivide), ctypes.byref(w)))
+        return w.value
 
     def timing(self, enable: bool):
         check(lib.dpx_batch_timing(self.h, int(enable)))

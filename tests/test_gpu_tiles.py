@@ -1,9 +1,10 @@
-"""Every raster tile shape (16x16, 32x8 and 64x4 cells per block) against
-the oracle: random batches (random kinds, sizes including items narrower
-than a tile and ragged edges, scales 1..15, subdivision, layouts) with the
-context's tile picks forced to one shape for every size class and scale,
-then the workload shapes (sprites, console frames, a large frame) whose
-picks the autotuner chooses between."""
+"""Every raster tile shape (16x16, 32x8 and 64x4 cells per block), with and
+without the walk window, against the oracle: random batches (random kinds,
+sizes including items narrower than a tile and ragged edges, scales 1..15,
+subdivision, layouts) with the context's picks forced to one shape and one
+walk mode for every size class and scale, then the workload shapes
+(sprites, console frames, a large frame) whose picks the autotuner chooses
+between."""
 import numpy as np
 import pytest
 
@@ -12,11 +13,13 @@ from test_gpu_random import SENTINEL, random_batch
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(300)]
 
 
-def force_tile(D, ctx, tx):
+def force_tile(D, ctx, tx, walk=None):
     for cls in (D.CLASS_SPRITE, D.CLASS_FRAME, D.CLASS_LARGE):
         for s in range(1, 16):
             for sub in (False, True):
                 ctx.set_tile(cls, s, sub, tx)
+                if walk is not None:
+                    ctx.set_walk(cls, s, sub, walk)
 
 
 def check_batch(torch, D, O, ctx, items, blob, out_bytes, meta, label):
@@ -38,15 +41,17 @@ def check_batch(torch, D, O, ctx, items, blob, out_bytes, meta, label):
     assert np.all(out[~touched] == SENTINEL), f"{label}: bytes outside the images were written"
 
 
+@pytest.mark.parametrize("walk", [1, 2])  # DPX_WALK_ALWAYS, DPX_WALK_NEVER
 @pytest.mark.parametrize("tx", [16, 32, 64])
 @pytest.mark.parametrize("seed", [0, 1, 2, 3])
-def test_tile_shape_random_batches(gpu, D, O, tx, seed):
+def test_tile_shape_random_batches(gpu, D, O, tx, seed, walk):
     torch = gpu
-    print(f"tile {tx} batch seed {7100 + seed}", flush=True)
+    print(f"tile {tx} walk mode {walk} batch seed {7100 + seed}", flush=True)
     ctx = D.Context(0)
-    force_tile(D, ctx, tx)
+    force_tile(D, ctx, tx, walk)
+    assert ctx.get_walk(D.CLASS_FRAME, 3, True) == walk
     items, blob, out_bytes, meta = random_batch(D, 7100 + seed)
-    check_batch(torch, D, O, ctx, items, blob, out_bytes, meta, f"tile {tx} seed {7100 + seed}")
+    check_batch(torch, D, O, ctx, items, blob, out_bytes, meta, f"tile {tx} walk {walk} seed {7100 + seed}")
     ctx.close()
 
 
@@ -67,12 +72,13 @@ def workload_batch(D):
     return items, np.concatenate(blobs), out_off, meta
 
 
+@pytest.mark.parametrize("walk", [1, 2])
 @pytest.mark.parametrize("tx", [16, 32, 64])
-def test_tile_shape_workload_items(gpu, D, O, tx):
+def test_tile_shape_workload_items(gpu, D, O, tx, walk):
     torch = gpu
-    print(f"tile {tx} workload items", flush=True)
+    print(f"tile {tx} walk mode {walk} workload items", flush=True)
     ctx = D.Context(0)
-    force_tile(D, ctx, tx)
+    force_tile(D, ctx, tx, walk)
     for cls in (D.CLASS_SPRITE, D.CLASS_FRAME, D.CLASS_LARGE):
         assert ctx.get_tile(cls, 4, False) == tx
     items, blob, out_bytes, meta = workload_batch(D)

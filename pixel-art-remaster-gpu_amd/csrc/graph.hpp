@@ -54,13 +54,51 @@ __host__ __device__ inline bool remove_slash(G get, int i, int j)
     bool v1 = o1 == 0, v4 = o4 == 0, v3 = o3 == 0, v2 = o2 == 0;
     if ((v1 || v4) && !v3 && !v2) return false;
     if (v3 || (v2 && !v1 && !v4)) return true;
-    // curve lengths; ties remove "/"
-    int seg = 0, seg2 = 0;
-    val2_walk(get, i, j, 2, &seg);
-    val2_walk(get, i + 1, j + 1, 5, &seg);
-    val2_walk(get, i + 1, j, 0, &seg2);
-    val2_walk(get, i, j + 1, 7, &seg2);
-    return !(seg2 < seg);
+    // Curve lengths; ties remove "/".  The reference walks from each end of
+    // "/" into one accumulator, then from each end of "\" (val2_walk from
+    // (i,j) edge 2 and (i+1,j+1) edge 5, then (i+1,j) edge 0 and (i,j+1)
+    // edge 7).  A walk from a node with others != 1 takes no step, and past
+    // the curve rule at most one end of each diagonal has others == 1, so
+    // each length is one walk: min(steps, 31).  The two walks read only the
+    // snapshot, so they advance side by side (two independent reads per
+    // step), and stop once the comparison is settled: a finished length a
+    // against a running count b >= a (or > a) cannot change the answer.
+    int ax = i, ay = j, ae = 2, bx = i + 1, by = j, be = 0;
+    uint32_t ar = n1 & ~(1u << 2), br = n3 & ~(1u << 0);
+    bool da = false, db = false;
+    if (o1 != 1) {
+        ax = i + 1; ay = j + 1; ae = 5; ar = n4 & ~(1u << 5);
+        da = o4 != 1;
+    }
+    if (o3 != 1) {
+        bx = i; by = j + 1; be = 7; br = n2 & ~(1u << 7);
+        db = o2 != 1;
+    }
+    int seg = 0, seg2 = 0;  // "/" and "\" lengths so far
+    for (;;) {
+        if (da && seg2 >= seg) return true;
+        if (db && seg > seg2) return false;
+        if (!da) {
+            const int k = __builtin_ctz(ar);
+            ax += edx(k); ay += edy(k); ae = 7 - k;
+            seg++;
+        }
+        if (!db) {
+            const int k = __builtin_ctz(br);
+            bx += edx(k); by += edy(k); be = 7 - k;
+            seg2++;
+        }
+        // (a finished walk stays on an in-frame node: its read is harmless)
+        const uint32_t va = get(ax, ay), vb = get(bx, by);
+        if (!da) {
+            ar = va & ~(1u << ae);
+            da = seg > 30 || __builtin_popcount(ar) != 1;
+        }
+        if (!db) {
+            br = vb & ~(1u << be);
+            db = seg2 > 30 || __builtin_popcount(br) != 1;
+        }
+    }
 }
 
 // Diagonal bits (0,2,5,7) the pixel at (i,j) keeps after crossCheck_4

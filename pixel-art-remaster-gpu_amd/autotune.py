@@ -58,6 +58,7 @@ def workloads(D):
     for s in (2, 3, 6, 8):
         out.append((f"256 frames of 256x224 at {s}x", D.CLASS_FRAME, s, 0, frames[:256]))
     out.append(("1024 frames of 256x224 at 4x (the headline batch)", D.CLASS_FRAME, 4, 0, frames))
+    out.append(("64 console frames of five sizes at 4x", D.CLASS_FRAME, 4, 0, console))
     out.append(("64 frames of 256x224 at 4x, subdivided", D.CLASS_FRAME, 4, 1, frames[:64]))
     out.append(("64 console frames of five sizes at 4x, subdivided", D.CLASS_FRAME, 4, 1, console))
     for s in (4, 8):

@@ -21,24 +21,6 @@ __host__ __device__ inline int others(uint32_t node, int edge)
 // in-frame positions are asked for: walks follow set bits, which never
 // point off the frame.
 
-// calcVal2PathSize (graph_functions.cu:541-573) as a loop: follow valence-2
-// nodes away from `edge`, adding one per step to the shared accumulator
-// while it is <= 30 (so a diagonal's two walks sum to at most 31).
-template <class G>
-__host__ __device__ inline void val2_walk(G get, int x, int y, int edge, int* result)
-{
-    for (;;) {
-        uint32_t rest = get(x, y) & ~(1u << edge);
-        if (*result > 30) return;
-        if (__builtin_popcount(rest) != 1) return;
-        (*result)++;
-        int k = __builtin_ctz(rest);
-        x += edx(k);
-        y += edy(k);
-        edge = 7 - k;
-    }
-}
-
 // processHeuristics2 (graph_functions.cu:761-849) for the 2x2 block whose
 // lower-left pixel is (i,j): i1=(i,j) i2=(i,j+1) i3=(i+1,j) i4=(i+1,j+1).
 // true: remove "/" (i1-i4, bits 2 and 5); false: remove "\" (i2-i3, bits 0 and 7).
@@ -54,10 +36,12 @@ __host__ __device__ inline bool remove_slash(G get, int i, int j)
     bool v1 = o1 == 0, v4 = o4 == 0, v3 = o3 == 0, v2 = o2 == 0;
     if ((v1 || v4) && !v3 && !v2) return false;
     if (v3 || (v2 && !v1 && !v4)) return true;
-    // Curve lengths; ties remove "/".  The reference walks from each end of
-    // "/" into one accumulator, then from each end of "\" (val2_walk from
-    // (i,j) edge 2 and (i+1,j+1) edge 5, then (i+1,j) edge 0 and (i,j+1)
-    // edge 7).  A walk from a node with others != 1 takes no step, and past
+    // Curve lengths; ties remove "/".  calcVal2PathSize (graph_functions.cu:
+    // 541-573) follows valence-2 nodes away from an edge, adding one per
+    // step to an accumulator while it is <= 30; the reference runs it from
+    // each end of "/" into one accumulator ((i,j) away from edge 2, then
+    // (i+1,j+1) away from edge 5), then from each end of "\" ((i+1,j) edge
+    // 0, (i,j+1) edge 7).  A walk from a node with others != 1 takes no step, and past
     // the curve rule at most one end of each diagonal has others == 1, so
     // each length is one walk: min(steps, 31).  The two walks read only the
     // snapshot, so they advance side by side (two independent reads per

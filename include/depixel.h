@@ -242,10 +242,10 @@ int dpx_context_get_tile(dpx_context* ctx, int size_class, int scale, int subdiv
  * halo: with it a tile loads 32 more bytes and rows of the graph snapshot on
  * each side into LDS before deciding, so that no curve walk reads HBM one
  * step at a time; without it walks that leave the tile's small window do.
- * DPX_WALK_AUTO (the default): for launches of at most 2048 blocks only,
- * where a tile's walk latency is the launch's; DPX_WALK_ALWAYS: every
- * launch (picked by the autotuner where it is faster); DPX_WALK_NEVER.
- * The results are the same in every mode. */
+ * DPX_WALK_AUTO (the default): the tuned choice, i.e. the window where the
+ * autotuner measured it faster (tuning.json), else none; DPX_WALK_ALWAYS:
+ * every launch of the key; DPX_WALK_NEVER: none.  The results are the same
+ * in every mode. */
 #define DPX_WALK_AUTO   0
 #define DPX_WALK_ALWAYS 1
 #define DPX_WALK_NEVER  2

@@ -9,8 +9,8 @@ HBM) is timed on the device with the context's own kernel timing (HIP
 events on the launch stream), the outputs of all six are checked to be
 identical, and the fastest per (size class, scale, subdivision) is written
 to tuning.json next to this file, which every dpx_context reads when it is
-created.  A choice replaces the default (32x8, no walk window for launches
-of more than 2048 blocks) only when it is faster by more than `--margin`.
+created.  A choice replaces the default (32x8, no walk window) only when it
+is faster by more than `--margin`.
 
     python pixel-art-remaster-gpu_amd/autotune.py [--out tuning.json] [--runs 10]
 

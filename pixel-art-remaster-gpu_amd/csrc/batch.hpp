@@ -198,10 +198,10 @@ __device__ inline void tile_snapshot_load(const DevItem& it, const uint8_t* __re
 // the kernel uses only later): kWalkEX more bytes on each side and kWalkEY
 // more rows above and below, so that no walk (at most 31 steps from a node
 // next to a block of the small window) leaves it.  It costs every such tile
-// 7-10 KB more reads, which
-// a full batch pays in HBM time (15% of the 256x224 batch), but a batch of a
-// few frames is bound by the latency of its slowest tile's walks instead:
-// the host picks WALK for small launches.
+// 7-10 KB more reads, which a full batch pays in HBM time (20% of the
+// 256x224 batch); it pays where dense crossing lattices make long walks the
+// tile's critical path (640x480 frames at 8x), and the autotuner picks it
+// per workload class.
 constexpr int kWalkEX = 32, kWalkEY = 32;
 template <int TX>
 struct WalkWindow {

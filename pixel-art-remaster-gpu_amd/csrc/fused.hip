@@ -291,14 +291,8 @@ struct PlanGroup {
     dim3 grid;
     int grid3d;
     size_t off_items, off_map;  // in the descriptor buffer
-    bool walk;  // the crossing walks read a larger LDS window (small launches)
+    bool walk;  // the crossing walks read a larger LDS window (tile_resolve's WALK)
 };
-// Launches of at most this many blocks resolve crossings with the larger
-// walk window (tile_resolve's WALK): a grid of about one block per CU runs
-// as long as its slowest tile, whose curve walks step through HBM beyond the
-// small window one dependent read at a time; larger grids hide that latency
-// behind other blocks and would pay the window's extra reads in HBM time.
-constexpr uint64_t kWalkWindowBlocks = 2048;
 struct BatchPlan {
     bool valid = false;
     std::vector<dpx_item> items;
@@ -331,8 +325,9 @@ struct dpx_context {
     bool last_kept = false;   // the last batch wrote them
     // raster tile width per (size class, scale, subdivision); 0: the default
     int8_t tile_pick[3][dpx::kMaxScale + 1][2] = {};
-    // walk window per (size class, scale, subdivision): DPX_WALK_AUTO (small
-    // launches only), DPX_WALK_ALWAYS or DPX_WALK_NEVER
+    // walk window per (size class, scale, subdivision): DPX_WALK_AUTO (the
+    // tuned choice: DPX_WALK_ALWAYS where tuning.json asks for the window,
+    // else none), DPX_WALK_ALWAYS or DPX_WALK_NEVER
     int8_t walk_pick[3][dpx::kMaxScale + 1][2] = {};
     std::mutex mu;
     // kernel timing: 3 events per recorded batch (start, after graph, end)
@@ -766,9 +761,7 @@ static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* it
         const Group& grp = kv.second;
         size_t a = put(grp.dev.data(), grp.dev.size() * sizeof(DevItem));
         size_t b = put(grp.map.data(), grp.map.size() * sizeof(uint2));
-        const bool walk = grp.walk == DPX_WALK_ALWAYS ||
-                          (grp.walk == DPX_WALK_AUTO &&
-                           (uint64_t)grp.grid.x * grp.grid.y * grp.grid.z <= kWalkWindowBlocks);
+        const bool walk = grp.walk == DPX_WALK_ALWAYS;
         plan.groups.push_back(PlanGroup{ grp.scale, grp.sub, grp.tx, grp.grid, grp.grid3d, a, b, walk });
     }
     plan.items.assign(items, items + n_items);

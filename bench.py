@@ -360,6 +360,32 @@ def main():
         ok = [s for s in sweep if s["p99_ms"] < P99_BUDGET_MS]
         best_b = max(ok or sweep, key=lambda s: s["frames_per_s"])["batch"]
 
+    # small batches: steps issued back to back without per-step events (the
+    # sweep's event records bound it by the host at B = 1), and one step
+    # alone from submit to completion
+    small = []
+    if not args.batch:
+        for B in (1, 2, 4, 8):
+            items = D.items_array(items_all[:B])
+            for _ in range(20):
+                ctx.run(d_in, d_out, items, stream)
+            torch.cuda.synchronize()
+            n = 2000
+            t0 = time.perf_counter()
+            for _ in range(n):
+                ctx.run(d_in, d_out, items, stream)
+            torch.cuda.synchronize()
+            back = (time.perf_counter() - t0) / n * 1e6
+            one = 1e30
+            for _ in range(200):
+                t1 = time.perf_counter()
+                ctx.run(d_in, d_out, items, stream)
+                torch.cuda.synchronize()
+                one = min(one, (time.perf_counter() - t1) * 1e6)
+            small.append({"batch": B, "us_per_step_back_to_back": round(back, 2), "us_per_frame": round(back / B, 2),
+                          "us_one_step_alone_best_of_200": round(one, 2)})
+            log(f"B={B}: {back:.2f} us per step back to back, {one:.2f} us for one step alone")
+
     log(f"timed run: B={best_b}, {args.steps} steps")
     wall, lat, times = run_steps(best_b, args.steps, args.warmup)
     frames = best_b * args.steps
@@ -517,6 +543,7 @@ def main():
         "kernel_ms": {"graph": round(graph_ms, 5), "raster": round(raster_ms, 5)},
         "parity_spot_check": parity_ok,
         "sweep": sweep,
+        "small_batches": small,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": f"kf_raster<4,{'true' if sub else 'false'}>",

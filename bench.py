@@ -273,8 +273,38 @@ def bench_components(args, torch, D):
             res[f"{name}_{W}x{H}_ms"] = round(ms, 4)
         res[f"chains_{W}x{H}"] = int(counts[0].item())
         res[f"chain_nodes_{W}x{H}"] = int(counts[1].item())
-    print(json.dumps({"metric": "F1 stage times (components, border chains)", "unit": "ms per frame",
-                      "n_gpus": 1, "data": "synthetic", **res}), flush=True)
+        if not args.no_cpu:
+            # CPU baseline: the oracle's C routines (the reference's sequential
+            # algorithms) on the same graph, one host thread, best of 5
+            sys.path.insert(0, ROOT)
+            import oracle as O
+            ga = np.ascontiguousarray(g, dtype=np.uint8)
+            lab_h = np.zeros(W * H, dtype=np.int32)
+            nodes_h = np.zeros(8 * W * H + 8, dtype=np.int32)
+            sizes_h = np.zeros(W * H + 1, dtype=np.int32)
+            nc, nn = ctypes.c_int(0), ctypes.c_int(0)
+            cpu = {
+                "components": lambda: O.lib().orc_components(O._p(ga), W, H, O._p(lab_h)),
+                "border_chains": lambda: O.lib().orc_border_chains(O._p(ga), W, H, O._p(nodes_h), O._p(sizes_h),
+                                                                   nodes_h.size, sizes_h.size, ctypes.byref(nc),
+                                                                   ctypes.byref(nn)),
+            }
+            for name, fn in cpu.items():
+                best = 1e30
+                for _ in range(5):
+                    t0 = time.perf_counter()
+                    rc = fn()
+                    best = min(best, time.perf_counter() - t0)
+                    assert rc == 0, f"oracle {name}: {rc}"
+                res[f"cpu_{name}_{W}x{H}_ms"] = round(best * 1e3, 4)
+            # the same chains as the GPU's (counts)
+            res[f"cpu_chains_match_{W}x{H}"] = nc.value == res[f"chains_{W}x{H}"] and nn.value == res[f"chain_nodes_{W}x{H}"]
+    out = {"metric": "F1 stage times (components, border chains)", "unit": "ms per frame", "n_gpus": 1,
+           "data": "synthetic", **res}
+    if not args.no_cpu:
+        out["cpu_baseline"] = {"kind": "port", "cores": 1, "unit": "ms per frame",
+                               "sample": "the oracle's C components and border chains on the same graphs, best of 5"}
+    print(json.dumps(out), flush=True)
 
 
 def main():

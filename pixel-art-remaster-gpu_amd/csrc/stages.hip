@@ -791,6 +791,7 @@ __global__ __launch_bounds__(256) void k_bc_scan(const uint8_t* __restrict__ g, 
                 const int n = lane_of((int)r.w, bs[a]) ? L - 1 : L - 1 - pos;
 #pragma unroll
                 for (int u = 0; u < kBcU; u++) {
+                    if (64 * u >= n) break;  // (uniform) most walks fill one slot of 64
                     const int k = 1 + lane + 64 * u;
                     s_pre[a][lane + 64 * u] = k <= n ? walk_node(off, L, pos, k) : 0;
                 }
@@ -808,15 +809,17 @@ __global__ __launch_bounds__(256) void k_bc_scan(const uint8_t* __restrict__ g, 
                 bool hit = false;
                 for (int c = 0; c < n; c += kBcChunk) {
                     uint64_t bal[kBcU];
+                    const int rem = n - c;  // (uniform) slots of 64 past it are skipped
 #pragma unroll
                     for (int u = 0; u < kBcU; u++) {
                         const int k = c + 1 + lane + 64 * u;
-                        nd[u] = k <= n ? (c == 0 ? s_pre[a][lane + 64 * u] : walk_node(off, L, pos, k)) : 0;
+                        nd[u] = 64 * u < rem && k <= n ? (c == 0 ? s_pre[a][lane + 64 * u] : walk_node(off, L, pos, k))
+                                                       : 0;
                     }
 #pragma unroll
                     for (int u = 0; u < kBcU; u++) {
                         const int k = c + 1 + lane + 64 * u;
-                        bal[u] = __ballot(k <= n && (get(listed, nd[u]) | get(disc, nd[u])));
+                        bal[u] = 64 * u < rem ? __ballot(k <= n && (get(listed, nd[u]) | get(disc, nd[u]))) : 0ull;
                     }
                     // place in the chunk of the node before the first marked
                     // one (-1: the node before the chunk), else of the last
@@ -856,6 +859,7 @@ __global__ __launch_bounds__(256) void k_bc_scan(const uint8_t* __restrict__ g, 
                     }
 #pragma unroll
                     for (int u = 0; u < kBcU; u++) {
+                        if (64 * u >= n) break;  // (uniform)
                         const int k = 1 + lane + 64 * u;
                         if (k <= n) {
                             nodes[nn + k] = nd[u];

@@ -382,6 +382,7 @@ __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ 
     const int8_t* s_hull = reinterpret_cast<const int8_t*>(s_hull32);
     __shared__ uint16_t s_list[RTX * RTY];
     __shared__ int s_nlist;
+    __shared__ int s_bin[16];  // own cells to smooth per hull size, then the first list place of each size
     // point k of lane t at [k * 256 + t]: x | y << 16 (1/64 units); a hull has
     // at most 8 vertices (the table's), its smoothed polygon at most 16
     __shared__ uint32_t s_poly[16 * 256];
@@ -397,6 +398,7 @@ __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ 
         s_nlist = 0;
         s_nblk = 0;
     }
+    if (t < 16) s_bin[t] = 0;
     if (t < (GY + 1) * RWW) s_p32[t] = 0;
     tile_snapshot_load<TX>(it, aux, x0, ay0, s_aux32, t);
     __syncthreads();
@@ -414,12 +416,31 @@ __global__ __launch_bounds__(256, 4) void kf_smooth(const DevItem* __restrict__ 
 #pragma unroll
         for (int q = 0; q < HB / 4; q++) s_hull32[k * (HB / 4) + q] = e[q];
     }
-    // own cells to smooth: in the frame and not internal
+    // own cells to smooth: in the frame and not internal, listed by the size
+    // of their hull (a cell's work grows with it), largest first, so that
+    // the lanes of a wave smooth polygons of about the same size
+    int my_n = -1, my_rank = 0;
     {
         const int cx = t % RTX, cy = t / RTX;
-        const uint32_t node = s_g[(cy + 1) * RWB + cx + 8];
-        if (x0 + cx < it.W && y0 + cy < it.H && node != 90) s_list[atomicAdd(&s_nlist, 1)] = (uint16_t)t;
+        const int c = (cy + 1) * RWB + cx + 8;
+        const uint32_t node = s_g[c];
+        if (x0 + cx < it.W && y0 + cy < it.H && node != 90) {
+            my_n = hull_table[(size_t)cell_key(node, s_g[c - 1], s_g[c + 1]) * kHullBytes];
+            my_rank = atomicAdd(&s_bin[my_n], 1);
+        }
     }
+    __syncthreads();
+    if (t == 0) {
+        int acc = 0;
+        for (int b = 15; b >= 0; b--) {
+            const int v = s_bin[b];
+            s_bin[b] = acc;
+            acc += v;
+        }
+        s_nlist = acc;
+    }
+    __syncthreads();
+    if (my_n >= 0) s_list[s_bin[my_n] + my_rank] = (uint16_t)t;
     __syncthreads();
     if (t >= s_nlist) return;
     const int own = s_list[t];

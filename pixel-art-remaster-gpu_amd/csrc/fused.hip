@@ -1,7 +1,7 @@
 // fused.hip -- the batched path: many frames or sprites of uneven size
 // through two fused kernels.
 //
-//   kf_graph   tile of 64x32 pixels: lane words (YUV fields + class) once
+//   kf_graph   tile of 64x32 pixels (64x8 in small launches): lane words (YUV fields + class) once
 //              per pixel, similarity in four directions (the other four by
 //              symmetry) with block flags for the trivial crossing removal
 //              -> snapshot graph (1 B/px)
@@ -147,22 +147,31 @@ __global__ void kt_info(const uint32_t* __restrict__ mask, int S, uint16_t* __re
 //      edges, the position above it its right edge and the position right
 //      of it its up edge; the flags of the four blocks around a pixel come
 //      from those bits of the neighbouring bytes.
-constexpr int GTX = 64, GTY = 32;
-constexpr int GLX = GTX + 3, GLY = GTY + 4;  // lane words
-constexpr int GHX = GTX + 2, GHY = GTY + 2;  // positions of pass 2
-constexpr int GHP = GTX + 8;                 // byte pitch of pass 2's rows: column c is x = x0 - 4 + c
+// Tiles are GTX x TY pixels: TY = GTY, or kGraphSmallTY for launches that
+// would have fewer than kGraphSmallTiles blocks of GTY rows (a launch of a
+// few frames then spreads over more CUs; the extra halo rows cost
+// throughput only in launches that fill the machine).
+constexpr int GTX = 64, GTY = 32, kGraphSmallTY = 8;
+constexpr uint64_t kGraphSmallTiles = 512;
+constexpr int GLX = GTX + 3;  // lane words per row
+constexpr int GHX = GTX + 2;  // positions of pass 2 per row
+constexpr int GHP = GTX + 8;  // byte pitch of pass 2's rows: column c is x = x0 - 4 + c
 
+template <int TY>
 __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ items, const uint2* __restrict__ tile_map,
                                                 int grid3d, const uint8_t* __restrict__ in,
                                                 uint8_t* __restrict__ gaux)
 {
+    constexpr int GLY = TY + 4, GHY = TY + 2;  // rows of lane words and of pass 2's positions
+    static_assert(GLY * (GLX - 64) <= 256 && GHY * (GHX - 64) <= 256, "the columns past 64 take one pass");
     __shared__ uint32_t s_lane[GLX * GLY];
-    __shared__ uint32_t s_h32[GHY * GHP / 4];
+    // pass 2's rows; first, for indexed items, the 256 palette lane words
+    __shared__ uint32_t s_h32[GHY * GHP / 4 > 256 ? GHY * GHP / 4 : 256];
     uint8_t* s_h = reinterpret_cast<uint8_t*>(s_h32);
     const int t = threadIdx.x;
     const TileAt at = locate_tile<1>(tile_map, grid3d);
     const DevItem it = items[at.item];
-    const int x0 = at.tx * GTX, y0 = at.ty * GTY;
+    const int x0 = at.tx * GTX, y0 = at.ty * TY;
 
     // pass 1.  Byte offsets within an item fit 31 bits (checked on the host).
     const uint8_t* base = in + it.in_off;
@@ -238,7 +247,7 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
     __syncthreads();
     // pass 3
     uint8_t* g = gaux + it.g_off;
-    for (int k = t; k < GTX / 4 * GTY; k += 256) {
+    for (int k = t; k < GTX / 4 * TY; k += 256) {
         int q = k % (GTX / 4), ly = k / (GTX / 4);
         int x = x0 + 4 * q, y = y0 + ly;
         if (x >= it.W || y >= it.H) continue;
@@ -274,6 +283,9 @@ __global__ __launch_bounds__(256) void kf_graph(const DevItem* __restrict__ item
     }
 }
 
+using GraphFn = void (*)(const DevItem*, const uint2*, int, const uint8_t*, uint8_t*);
+inline GraphFn graph_kernel(int ty) { return ty == kGraphSmallTY ? kf_graph<kGraphSmallTY> : kf_graph<GTY>; }
+
 }  // namespace dpx
 
 
@@ -299,6 +311,7 @@ struct BatchPlan {
     uintptr_t out_mod = 0;  // d_out % 16
     dim3 ggrid;
     int g3d = 0;
+    int gty = dpx::GTY;  // graph tile rows (kf_graph<gty>)
     size_t off_items = 0, off_gmap = 0;
     std::vector<PlanGroup> groups;
 };
@@ -740,7 +753,10 @@ static int build_plan(dpx_context* ctx, const uint8_t* d_out, const dpx_item* it
     // launch grids
     int g3d = 0;
     std::vector<uint2> gmap;
-    const dim3 ggrid = tile_grid(dev, GTX, GTY, &g3d, &gmap);
+    uint64_t gtiles = 0;
+    for (const DevItem& d : dev) gtiles += (uint64_t)((d.W + GTX - 1) / GTX) * ((d.H + GTY - 1) / GTY);
+    plan.gty = gtiles < kGraphSmallTiles ? kGraphSmallTY : GTY;
+    const dim3 ggrid = tile_grid(dev, GTX, plan.gty, &g3d, &gmap);
     for (auto& kv : groups)
         kv.second.grid = tile_grid(kv.second.dev, kv.second.tx, 256 / kv.second.tx, &kv.second.grid3d, &kv.second.map);
 
@@ -816,7 +832,8 @@ static int launch_plan(dpx_context* ctx, const BatchPlan& plan, const LaunchArgs
 {
     const DevItem* d_items = reinterpret_cast<const DevItem*>(la.desc + plan.off_items);
     const uint2* d_gmap = reinterpret_cast<const uint2*>(la.desc + plan.off_gmap);
-    hipLaunchKernelGGL(kf_graph, plan.ggrid, dim3(256), 0, stream, d_items, d_gmap, plan.g3d, d_in, la.gaux);
+    hipLaunchKernelGGL(graph_kernel(plan.gty), plan.ggrid, dim3(256), 0, stream, d_items, d_gmap, plan.g3d, d_in,
+                       la.gaux);
     if (ev_mid) DPX_HIP(hipEventRecord(ev_mid, stream));
     for (const PlanGroup& pg : plan.groups) {
         RasterFn fn = pick_raster(pg.scale, pg.sub, pg.tx, pg.walk);
@@ -941,7 +958,7 @@ hipKernelNodeParams node_params(dpx_plan* p, int kind, int group, const uint8_t*
         a.g3d = p->plan.g3d;
         void* v[5] = { &a.items, &a.map, &a.g3d, &a.in, &a.gaux };
         memcpy(a.ptrs, v, sizeof(v));
-        np.func = reinterpret_cast<void*>(kf_graph);
+        np.func = reinterpret_cast<void*>(graph_kernel(p->plan.gty));
         np.gridDim = p->plan.ggrid;
     } else {
         const PlanGroup& pg = p->plan.groups[group];
@@ -1303,7 +1320,7 @@ extern "C" int dpx_batch_run_noalloc(const uint8_t* d_in, uint8_t* d_out, const 
             args.n = 0;
         }
     }
-    hipLaunchKernelGGL(kf_graph, dim3((unsigned)graph_tiles), dim3(256), 0, stream, d_items, d_gmap, 0, d_in,
+    hipLaunchKernelGGL(kf_graph<GTY>, dim3((unsigned)graph_tiles), dim3(256), 0, stream, d_items, d_gmap, 0, d_in,
                        d_graph);
     for (int g = 0; g < 2 * (kMaxScale + 1); g++) {
         if (!gcount[g]) continue;

@@ -115,7 +115,8 @@ int dpx_stage_similarity_graph(const uint8_t* d_img, int width, int height, int 
 /* trivial_cross_Kernel kernel.cu:162-177 (in place) */
 int dpx_stage_trivial_cross(uint8_t* d_graph, int width, int height, void* stream);
 /* snapshot kernel.cu:415 + ambiguous_cross_Kernel kernel.cu:180-189 (in place;
- * d_graph_aux receives the snapshot) */
+ * d_graph_aux receives the snapshot).  A link that leaves the frame (none in
+ * a graph from the stages above) leads a curve walk to a node without links. */
 int dpx_stage_heuristic_cross(uint8_t* d_graph, uint8_t* d_graph_aux, int width, int height,
                               void* stream);
 /* cells_Kernel kernel.cu:192-213 */

@@ -17,9 +17,10 @@ __host__ __device__ inline int others(uint32_t node, int edge)
 }
 
 // The graph is read through an accessor get(x, y) -> byte, so a kernel can
-// serve the bytes of its own tile from LDS and the rest from HBM.  Only
-// in-frame positions are asked for: walks follow set bits, which never
-// point off the frame.
+// serve the bytes of its own tile from LDS and the rest from HBM.  Walks
+// follow set bits, which in a graph from the similarity stage never point
+// off the frame; an accessor over a caller's graph answers 0 (no links)
+// outside it.
 
 // processHeuristics2 (graph_functions.cu:761-849) for the 2x2 block whose
 // lower-left pixel is (i,j): i1=(i,j) i2=(i,j+1) i3=(i+1,j) i4=(i+1,j+1).
@@ -72,7 +73,7 @@ __host__ __device__ inline bool remove_slash(G get, int i, int j)
             bx += edx(k); by += edy(k); be = 7 - k;
             seg2++;
         }
-        // (a finished walk stays on an in-frame node: its read is harmless)
+        // (a finished walk stays where it stopped: reading there again is harmless)
         const uint32_t va = get(ax, ay), vb = get(bx, by);
         if (!da) {
             ar = va & ~(1u << ae);

@@ -74,7 +74,12 @@ __global__ void k_heuristic(uint8_t* __restrict__ graph, const uint8_t* __restri
     gather_nb(aux, W, H, i, j, nb);
     size_t idx = (size_t)j * W + i;
     uint32_t c = aux[idx];
-    auto get = [=](int x, int y) -> uint32_t { return aux[(size_t)y * W + x]; };
+    // a caller's graph may hold links that leave the frame (the similarity
+    // stage never makes one): a walk reaching past the edge finds a node
+    // without links there and stops
+    auto get = [=](int x, int y) -> uint32_t {
+        return (unsigned)x < (unsigned)W && (unsigned)y < (unsigned)H ? aux[(size_t)y * W + x] : 0u;
+    };
     graph[idx] = (uint8_t)(c & ~heuristic_clear_mask(get, i, j, c, nb));
 }
 

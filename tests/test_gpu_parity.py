@@ -8,6 +8,7 @@ import pytest
 pytestmark = [pytest.mark.gpu, pytest.mark.timeout(300)]
 
 CS = 45
+DX8, DY8 = [-1, 0, 1, -1, 1, -1, 0, 1], [1, 1, 1, 0, 0, -1, -1, -1]  # link e: (dx, dy)
 
 # (kind, seed, W, H, channels, flags, scale, subdivide, row padding)
 FRAME_CASES = [
@@ -129,6 +130,46 @@ def test_stage_entry_points_match_oracle(gpu, D, O):
     D.check(D.lib.dpx_stage_components(d_g.data_ptr(), W, H, d_lab.data_ptr(), s.cuda_stream))
     torch.cuda.synchronize()
     assert_equal(d_lab.cpu().numpy().reshape(H, W), O.components(g2), "component labels")
+
+
+def test_heuristic_stage_on_graphs_with_links_off_the_frame(gpu, D, O):
+    """A caller's graph may hold links that leave the frame (the similarity
+    stage never makes one): the curve walks of the heuristic stage then find
+    a node without links past the edge, as in the same graph inside a border
+    of nodes without links (the oracle's run on the padded graph, cropped).
+    Border pixels with one link get a second one leading off the frame, so
+    that walks along line-heavy frames' curves step past the edge."""
+    torch = gpu
+    dev = "cuda:0"
+    s = torch.cuda.current_stream()
+    stepped = changed = 0
+    for seed in range(12):
+        W, H = 24 + 3 * seed, 20 + 2 * seed
+        img = D.synth_frame(2, 4300 + seed, W, H, 3)
+        g = O.trivial_cross(O.similarity_graph(img, W, H)).copy()
+        g0 = g.copy()
+        rng = np.random.default_rng(seed)
+        for j in range(H):
+            for i in range(W):
+                if bin(int(g[j, i])).count("1") != 1:
+                    continue
+                bits = [e for e in range(8) if not (0 <= i + DX8[e] < W and 0 <= j + DY8[e] < H)]
+                if bits:
+                    g[j, i] |= np.uint8(1 << int(rng.choice(bits)))
+                    stepped += 1
+        pad = np.zeros((H + 2, W + 2), dtype=np.uint8)
+        pad[1:-1, 1:-1] = g
+        want = O.heuristic_cross(pad)[1:-1, 1:-1]
+        pad[1:-1, 1:-1] = g0
+        changed += int(((want ^ g) != (O.heuristic_cross(pad)[1:-1, 1:-1] ^ g0)).any())
+        d_g = torch.from_numpy(g.reshape(-1).copy()).to(dev)
+        d_ga = torch.zeros(W * H, dtype=torch.uint8, device=dev)
+        print("frame", seed, W, H, flush=True)
+        D.check(D.lib.dpx_stage_heuristic_cross(d_g.data_ptr(), d_ga.data_ptr(), W, H, s.cuda_stream))
+        torch.cuda.synchronize()
+        assert_equal(d_g.cpu().numpy().reshape(H, W), want, f"heuristic cross, frame {seed}")
+    # links were added, and walks past the edge changed some decisions
+    assert stepped > 50 and changed > 0
 
 
 def test_subdivide_stage_leaves_oversized_cells(gpu, D, O):

@@ -150,7 +150,7 @@ int dpx_stage_components(const uint8_t* d_graph, int width, int height, int32_t*
  * of d_sizes[k] node indices j*width+i, chains in the order the row-major scan
  * starts them.  d_scratch: width*height bytes, 4-byte aligned (it holds the
  * scan's marks when they do not fit a block's LDS, frames of more than about
- * 650,000 nodes).  Working memory of about 300 bytes per node is allocated
+ * 620,000 nodes).  Working memory of about 300 bytes per node is allocated
  * and freed in stream order (hipMallocAsync / hipFreeAsync on `stream`).
  * d_counts[3] receives the number of chains, of nodes and a status (0;
  * DPX_ENOMEM when max_nodes or max_chains is too small; DPX_EINVAL, with no

@@ -703,22 +703,54 @@ __global__ __launch_bounds__(256) void k_bc_scatter(const uint8_t* __restrict__ 
     if ((int)(id & 7u) == cw_first_link(g[id >> 3])) rec[id >> 3] = make_uint4(inf.x, inf.y, pos, inf.z);
 }
 
-// The scan, one wave (wave 0 of the block).  Marks are two bitmaps
-// (listed, discarded) of (N + 31) / 32 words each, in LDS when they fit,
-// else in bits_g.  What does not depend on the marks is fetched ahead: the
-// other three waves read the path arrays and the graph once, so that the
-// scan's reads of them hit the L2 of its XCD; the next two groups' records
-// are loaded while a group is scanned; the first kBcChunk nodes of the next
-// kBcAhead walks are loaded at once (a walk's nodes do not depend on the
-// marks, only its outcome does).  A walk's nodes are tested kBcChunk at a
-// time (kBcU per lane).  With the loads ahead, memory waits are a few
-// percent of the scan's cycles: its time is one wave's instruction stream
-// (about 300 instructions per walk, with no other wave to hide their
-// latencies), so deeper prefetching, a third bitmap and staging the output
-// in LDS all measured no faster or slower.
-constexpr int kBcU = 4, kBcChunk = 64 * kBcU, kBcAhead = 4;
+// The scan, one wave (wave 0 of the block).  Marks are two bitmaps of
+// (N + 31) / 32 words each, in LDS when they fit, else in bits_g: listed
+// (nodes of chains: a candidate that is listed starts no walk) and marked
+// (listed or discarded: a walk stops at such a node), so that a walk's test
+// reads one bit per node.  Which walks run depends on the marks, but what a walk
+// reads does not, so the loads go out in groups and the wave waits once per
+// group, not once per load:
+// - the candidate flags and walk records of kBcBlk batches of 64 nodes are
+//   loaded into registers while the block before is scanned, then staged
+//   in LDS;
+// - the first kBcChunk nodes of the next kBcAhead walks of a batch are
+//   loaded together into registers (a walk whose start a chain lists
+//   meanwhile is skipped);
+// - the rest of a long walk, and a long chain's nodes, kBcChunk2 at a time.
+// The loads are unconditional (an index past an end is clamped, its value
+// dropped): a load that the wave may skip leaves the compiler unsure how
+// many are in flight, and it then waits for all of them at the next use.
+// The other three waves read the path arrays, the graph and the records
+// once, so that the scan's loads hit the L2 of its XCD.  The sizes are the
+// fastest of a sweep on 256x224 and 640x480 frames (groups of 1-8 walks,
+// first chunks of 64-256 nodes, later ones of 128-1024, blocks of 4-16
+// batches): 85% of walks end within 64 nodes, and each load the wave
+// issues for a slot it may not need costs more than the wait it saves.
+constexpr int kBcU = 1, kBcChunk = 64 * kBcU, kBcAhead = 2, kBcBlk = 8, kBcU2 = 4, kBcChunk2 = 64 * kBcU2;
 
 __device__ inline int lane_of(int v, int l) { return __builtin_amdgcn_readlane(v, l); }
+
+// place of the node before the first marked one in a chunk of U slots of 64
+// (-1: the node before the chunk); -2 when no node of the chunk is marked
+template <int U>
+__device__ inline int bc_before_mark(const uint64_t (&bal)[U])
+{
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if (bal[u]) return 64 * u + __builtin_ctzll(bal[u]) - 1;
+    return -2;
+}
+
+// the node at place p (uniform) of a chunk held U per lane
+template <int U>
+__device__ inline int bc_node_at(const int (&nd)[U], int p)
+{
+    int x = 0;
+#pragma unroll
+    for (int u = 0; u < U; u++)
+        if ((p >> 6) == u) x = lane_of(nd[u], p & 63);
+    return x;
+}
 
 template <bool LDS>
 __global__ __launch_bounds__(256) void k_bc_scan(const uint8_t* __restrict__ g, int N, const uint4* __restrict__ rec,
@@ -727,20 +759,22 @@ __global__ __launch_bounds__(256) void k_bc_scan(const uint8_t* __restrict__ g, 
                                                  int32_t* counts)
 {
     extern __shared__ uint32_t s_bits[];
-    __shared__ int32_t s_pre[kBcAhead][kBcChunk];
+    __shared__ uint4 s_rec[kBcBlk][64];
+    __shared__ uint64_t s_cand[kBcBlk];
     uint32_t* bits = LDS ? s_bits : bits_g;
     const int NW = (N + 31) >> 5, tid = threadIdx.x, lane = tid & 63;
     uint32_t* listed = bits;
-    uint32_t* disc = bits + NW;
+    uint32_t* marked = bits + NW;
     for (int k = tid; k < 2 * NW; k += 256) bits[k] = 0u;
     if constexpr (!LDS) __threadfence();
     __syncthreads();
-    if (tid >= 64) {  // the helper waves: touch the path arrays and the graph, then leave
-        const volatile int32_t* vA = A;
-        const volatile uint8_t* vg = g;
+    if (tid >= 64) {  // the helper waves: read the path arrays, the graph and the records, then leave
         const int total = ctr[0] == 0 ? ctr[1] : 0;
-        for (int k = (tid - 64) * 16; k < total; k += 192 * 16) (void)vA[k];  // one read per 64 bytes
-        for (int k = (tid - 64) * 64; k < N; k += 192 * 64) (void)vg[k];
+        uint32_t acc = 0u;
+        for (int k = (tid - 64) * 16; k < total; k += 192 * 16) acc ^= (uint32_t)A[k];  // one read per 64 bytes
+        for (int k = (tid - 64) * 64; k < N; k += 192 * 64) acc ^= g[k];
+        for (int k = (tid - 64) * 4; k < N; k += 192 * 4) acc ^= rec[k].x;
+        asm volatile("" ::"v"(acc));  // (keeps the reads)
         return;
     }
     auto get = [&](const uint32_t* b, int n) -> uint32_t {
@@ -759,132 +793,167 @@ __global__ __launch_bounds__(256) void k_bc_scan(const uint8_t* __restrict__ g, 
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
         __builtin_amdgcn_wave_barrier();
     };
-    // node k (1-based, k <= n) of a walk from place pos of a path of L
-    auto walk_node = [&](int off, int L, int pos, int k) {
+    // node k (1-based) of a walk of n nodes from place pos of a path of L at
+    // off; past the walk (k > n) the path arrays' first entry, unused
+    auto walk_node = [&](int off, int L, int pos, int k, int n) -> int {
         int q = pos + k;
         if (q >= L) q -= L;
-        return A[off + q];
+        return A[k <= n ? off + q : 0];
     };
     int status = ctr[0], nc = 0, nn = 0;
-    uint32_t v1 = lane < N ? g[lane] : 90u, v2 = 64 + lane < N ? g[64 + lane] : 90u;
-    uint4 r1 = lane < N ? rec[lane] : make_uint4(0u, 0u, 0u, 0u);
-    uint4 r2 = 64 + lane < N ? rec[64 + lane] : make_uint4(0u, 0u, 0u, 0u);
-    for (int base = 0; base < N && status == 0; base += 64) {
-        const int me = base + lane;
-        const uint32_t v = v1;
-        const uint4 r = r1;
-        v1 = v2;
-        r1 = r2;
-        // (a record of a node that starts no walk is never used; nodes past
-        // the frame start none)
-        v2 = me + 128 < N ? g[me + 128] : 90u;
-        r2 = me + 128 < N ? rec[me + 128] : make_uint4(0u, 0u, 0u, 0u);
-        const bool cand = v != 90u && v != 0u;
-        uint64_t m = __ballot(cand && !get(listed, me));
-        while (m && status == 0) {
-            // the next walks of the group: their first chunks, all loads at once
-            int bs[kBcAhead];
+    // block k of kBcBlk batches: graph bytes and records into registers
+    // (nodes past the frame read its last node), then flags and records
+    // into LDS (the arrays stay in registers: both are inlined)
+    const int nblk = (N + 64 * kBcBlk - 1) / (64 * kBcBlk);
+    uint32_t vv[kBcBlk], rx[kBcBlk], ry[kBcBlk], rz[kBcBlk], rw[kBcBlk];  // (no uint4 array: it would go to scratch)
+    auto issue = [&](int k) __attribute__((always_inline)) {
 #pragma unroll
-            for (int a = 0; a < kBcAhead; a++) {
-                bs[a] = m ? __builtin_ctzll(m) : -1;
-                m &= m ? m - 1 : 0;
-            }
+        for (int j = 0; j < kBcBlk; j++) {
+            const int n = (k * kBcBlk + j) * 64 + lane, c = n < N ? n : N - 1;
+            const uint4 t = rec[c];
+            vv[j] = g[c];
+            rx[j] = t.x;
+            ry[j] = t.y;
+            rz[j] = t.z;
+            rw[j] = t.w;
+        }
+    };
+    auto stage = [&](int k) __attribute__((always_inline)) {
 #pragma unroll
-            for (int a = 0; a < kBcAhead; a++) {
-                if (bs[a] < 0) break;
-                const int off = lane_of((int)r.x, bs[a]), L = lane_of((int)r.y, bs[a]), pos = lane_of((int)r.z, bs[a]);
-                const int n = lane_of((int)r.w, bs[a]) ? L - 1 : L - 1 - pos;
+        for (int j = 0; j < kBcBlk; j++) {
+            const int n = (k * kBcBlk + j) * 64 + lane;
+            const uint64_t cm = __ballot(n < N && vv[j] != 90u && vv[j] != 0u);
+            if (lane == 0) s_cand[j] = cm;
+            s_rec[j][lane] = make_uint4(rx[j], ry[j], rz[j], rw[j]);
+        }
+    };
+    if (status == 0) issue(0);
+    for (int k = 0; k < nblk && status == 0; k++) {
+        stage(k);  // (the walks of block k - 1 are done with the LDS copy)
+        if (k + 1 < nblk) issue(k + 1);
+        for (int j = 0; j < kBcBlk && status == 0; j++) {
+            const int base = (k * kBcBlk + j) * 64;
+            if (base >= N) break;
+            const int me = base + lane;
+            const bool cand = (s_cand[j] >> lane) & 1u;
+            uint64_t m = __ballot(cand && !get(listed, me));
+            while (m && status == 0) {
+                // the next walks of the batch: their first chunks, all loads at once
+                int bs[kBcAhead], pv[kBcAhead][kBcU], ro[kBcAhead], rl[kBcAhead], rp[kBcAhead], rn[kBcAhead];
+                bool rc[kBcAhead];
 #pragma unroll
-                for (int u = 0; u < kBcU; u++) {
-                    if (64 * u >= n) break;  // (uniform) most walks fill one slot of 64
-                    const int k = 1 + lane + 64 * u;
-                    s_pre[a][lane + 64 * u] = k <= n ? walk_node(off, L, pos, k) : 0;
+                for (int a = 0; a < kBcAhead; a++) {
+                    bs[a] = m ? __builtin_ctzll(m) : -1;
+                    m &= m ? m - 1 : 0;
                 }
-            }
-            uint64_t alive = ~0ull;  // candidates not listed by a chain of this batch
-            for (int a = 0; a < kBcAhead && status == 0; a++) {
-                const int b = bs[a];
-                if (b < 0) break;
-                if (!((alive >> b) & 1u)) continue;
-                const int s = base + b;
-                const int off = lane_of((int)r.x, b), L = lane_of((int)r.y, b), pos = lane_of((int)r.z, b);
-                const bool cyc = lane_of((int)r.w, b) != 0;
-                const int n = cyc ? L - 1 : L - 1 - pos;  // walk nodes after the start
-                int prev = s, nd[kBcU];
-                bool hit = false;
-                for (int c = 0; c < n; c += kBcChunk) {
-                    uint64_t bal[kBcU];
-                    const int rem = n - c;  // (uniform) slots of 64 past it are skipped
 #pragma unroll
-                    for (int u = 0; u < kBcU; u++) {
-                        const int k = c + 1 + lane + 64 * u;
-                        nd[u] = 64 * u < rem && k <= n ? (c == 0 ? s_pre[a][lane + 64 * u] : walk_node(off, L, pos, k))
-                                                       : 0;
-                    }
-#pragma unroll
-                    for (int u = 0; u < kBcU; u++) {
-                        const int k = c + 1 + lane + 64 * u;
-                        bal[u] = 64 * u < rem ? __ballot(k <= n && (get(listed, nd[u]) | get(disc, nd[u]))) : 0ull;
-                    }
-                    // place in the chunk of the node before the first marked
-                    // one (-1: the node before the chunk), else of the last
-                    uint64_t any = 0;
-#pragma unroll
-                    for (int u = 0; u < kBcU; u++) any |= bal[u];
-                    hit = any != 0;
-                    int last = (n - c < kBcChunk ? n - c : kBcChunk) - 1;
-                    if (hit) {
-                        last = -1;
-#pragma unroll
-                        for (int u = kBcU - 1; u >= 0; u--)
-                            if (bal[u]) last = 64 * u + __builtin_ctzll(bal[u]) - 1;
-                    }
-                    if (last >= 0) {
-                        int x = 0;
-#pragma unroll
-                        for (int u = 0; u < kBcU; u++)
-                            if ((last >> 6) == u) x = lane_of(nd[u], last & 63);
-                        prev = x;
-                    }
-                    if (hit) break;
+                for (int a = 0; a < kBcAhead; a++) {  // their records: path offset, length, place, on a cycle
+                    const uint4 rb = s_rec[j][bs[a] < 0 ? 0 : bs[a]];
+                    ro[a] = (int)rb.x;
+                    rl[a] = (int)rb.y;
+                    rp[a] = (int)rb.z;
+                    rc[a] = rb.w != 0u;
+                    rn[a] = bs[a] < 0 ? 0 : rc[a] ? rl[a] - 1 : rl[a] - 1 - rp[a];  // walk nodes after the start
                 }
-                if (hit || !cyc) {  // abandoned (a path ends at an internal node)
-                    if (lane == 0) put(disc, prev);
-                    settle();
-                    continue;
-                }
-                if (nc >= max_chains || (int64_t)nn + L > max_nodes) {
-                    status = DPX_ENOMEM;
-                    break;
-                }
-                if (n < kBcChunk) {  // the whole walk is in nd
-                    if (lane == 0) {
-                        nodes[nn] = s;
-                        put(listed, s);
-                    }
 #pragma unroll
-                    for (int u = 0; u < kBcU; u++) {
-                        if (64 * u >= n) break;  // (uniform)
-                        const int k = 1 + lane + 64 * u;
-                        if (k <= n) {
-                            nodes[nn + k] = nd[u];
-                            put(listed, nd[u]);
+                for (int a = 0; a < kBcAhead; a++)
+#pragma unroll
+                    for (int u = 0; u < kBcU; u++)
+                        pv[a][u] = walk_node(ro[a], rl[a], rp[a], 1 + lane + 64 * u, rn[a]);
+                uint64_t alive = ~0ull;  // candidates not listed by a chain of this batch
+#pragma unroll
+                for (int a = 0; a < kBcAhead; a++) {  // (unrolled: pv[a] stays in registers)
+                    if (status != 0) break;
+                    const int b = bs[a];
+                    if (b < 0) break;
+                    if (!((alive >> b) & 1u)) continue;
+                    const int s = base + b;
+                    const int off = ro[a], L = rl[a], pos = rp[a], n = rn[a];
+                    const bool cyc = rc[a];
+                    int prev = s, nd[kBcU];
+                    bool hit;
+                    {  // the first chunk, loaded above
+                        uint64_t bal[kBcU];
+#pragma unroll
+                        for (int u = 0; u < kBcU; u++) nd[u] = pv[a][u];
+#pragma unroll
+                        for (int u = 0; u < kBcU; u++) {
+                            const int kk = 1 + lane + 64 * u;
+                            bal[u] = 64 * u < n ? __ballot(kk <= n && get(marked, nd[u])) : 0ull;
+                        }
+                        const int f = bc_before_mark(bal);
+                        hit = f != -2;
+                        const int last = hit ? f : (n < kBcChunk ? n : kBcChunk) - 1;
+                        if (last >= 0) prev = bc_node_at(nd, last);
+                    }
+                    for (int c = kBcChunk; !hit && c < n; c += kBcChunk2) {  // the rest of a long walk
+                        int nd2[kBcU2];
+                        uint64_t bal[kBcU2];
+#pragma unroll
+                        for (int u = 0; u < kBcU2; u++)  // (uniform)
+                            nd2[u] = walk_node(off, L, pos, c + 1 + lane + 64 * u, n);
+#pragma unroll
+                        for (int u = 0; u < kBcU2; u++) {
+                            const int kk = c + 1 + lane + 64 * u;
+                            bal[u] = c + 64 * u < n ? __ballot(kk <= n && get(marked, nd2[u])) : 0ull;
+                        }
+                        const int f = bc_before_mark(bal);
+                        hit = f != -2;
+                        const int last = hit ? f : (n - c < kBcChunk2 ? n - c : kBcChunk2) - 1;
+                        if (last >= 0) prev = bc_node_at(nd2, last);
+                    }
+                    if (hit || !cyc) {  // abandoned (a path ends at an internal node)
+                        if (lane == 0) put(marked, prev);
+                        settle();
+                        continue;
+                    }
+                    if (nc >= max_chains || (int64_t)nn + L > max_nodes) {
+                        status = DPX_ENOMEM;
+                        break;
+                    }
+                    if (n < kBcChunk) {  // the whole walk is in nd
+                        if (lane == 0) {
+                            nodes[nn] = s;
+                            put(listed, s);
+                            put(marked, s);
+                        }
+#pragma unroll
+                        for (int u = 0; u < kBcU; u++) {
+                            if (64 * u >= n) break;  // (uniform)
+                            const int kk = 1 + lane + 64 * u;
+                            if (kk <= n) {
+                                nodes[nn + kk] = nd[u];
+                                put(listed, nd[u]);
+                                put(marked, nd[u]);
+                            }
+                        }
+                    } else {
+                        for (int c = 0; c < L; c += kBcChunk2) {
+                            int v[kBcU2];
+#pragma unroll
+                            for (int u = 0; u < kBcU2; u++) {
+                                const int kk = c + lane + 64 * u;
+                                v[u] = c + 64 * u >= L ? 0 : kk == 0 ? s : walk_node(off, L, pos, kk, n);  // (uniform)
+                            }
+#pragma unroll
+                            for (int u = 0; u < kBcU2; u++) {
+                                const int kk = c + lane + 64 * u;
+                                if (kk < L) {
+                                    nodes[nn + kk] = v[u];
+                                    put(listed, v[u]);
+                                    put(marked, v[u]);
+                                }
+                            }
                         }
                     }
-                } else {
-                    for (int k = lane; k < L; k += 64) {
-                        const int node = k == 0 ? s : walk_node(off, L, pos, k);
-                        nodes[nn + k] = node;
-                        put(listed, node);
-                    }
+                    if (lane == 0) sizes[nc] = L;
+                    nc++;
+                    nn += L;
+                    settle();
+                    const uint64_t still = __ballot(cand && !get(listed, me));
+                    alive &= still;
+                    m &= still;
                 }
-                if (lane == 0) sizes[nc] = L;
-                nc++;
-                nn += L;
-                settle();
-                const uint64_t still = __ballot(cand && !get(listed, me));
-                alive &= still;
-                m &= still;
             }
         }
     }

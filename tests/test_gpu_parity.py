@@ -471,6 +471,17 @@ def test_border_chains_match_oracle(gpu, D, O):
         got, nn, status = gpu_border_chains(torch, D, g)
         assert status == 0
         assert got == O.border_chains(g), f"frame {kind}/{seed}"
+    # every orthogonal link of a 2000x200 grid: the frame's edge is one chain
+    # of 4,396 nodes, longer than the scan's output ring of 4,096
+    g = np.full((200, 2000), 90, dtype=np.uint8)
+    g[0, :] &= ~np.uint8(1 << 6)
+    g[-1, :] &= ~np.uint8(1 << 1)
+    g[:, 0] &= ~np.uint8(1 << 3)
+    g[:, -1] &= ~np.uint8(1 << 4)
+    print("grid 2000x200", flush=True)
+    got, nn, status = gpu_border_chains(torch, D, g)
+    ref = O.border_chains(g)
+    assert status == 0 and max(map(len, ref)) > 4096 and got == ref
 
 
 def random_symmetric_graph(seed):

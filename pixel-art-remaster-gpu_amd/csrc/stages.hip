@@ -717,9 +717,10 @@ __global__ __launch_bounds__(256) void k_bc_scatter(const uint8_t* __restrict__ 
 //   loaded together into registers (a walk whose start a chain lists
 //   meanwhile is skipped);
 // - the rest of a long walk, and a long chain's nodes, kBcChunk2 at a time.
-// The loads are unconditional (an index past an end is clamped, its value
-// dropped): a load that the wave may skip leaves the compiler unsure how
-// many are in flight, and it then waits for all of them at the next use.
+// The loads of records and of a walk's nodes are unconditional (an index
+// past an end is clamped, its value dropped): a load that the wave may skip
+// leaves the compiler unsure how many are in flight, and it then waits for
+// all of them at the next use.
 // The other three waves read the path arrays, the graph and the records
 // once, so that the scan's loads hit the L2 of its XCD.  The sizes are the
 // fastest of a sweep on 256x224 and 640x480 frames (groups of 1-8 walks,

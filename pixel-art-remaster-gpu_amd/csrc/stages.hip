@@ -653,17 +653,22 @@ __global__ __launch_bounds__(256) void k_bc_cut(const uint8_t* __restrict__ g, i
 
 // the first link of each path takes the path's range of the node array:
 // info[last link] = (offset, length, on a cycle)
-__global__ __launch_bounds__(256) void k_bc_heads(const uint4* __restrict__ F, uint4* __restrict__ info, int* ctr,
-                                                  uint32_t E)
+constexpr int kHeadsBlock = 1024;
+
+__global__ __launch_bounds__(kHeadsBlock) void k_bc_heads(const uint4* __restrict__ F, uint4* __restrict__ info,
+                                                          int* ctr, uint32_t E)
 {
-    const uint32_t id = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t id = blockIdx.x * (uint32_t)kHeadsBlock + threadIdx.x;
     const int lane = threadIdx.x & 63;
     const bool ok = id < E && ctr[0] == 0;
     const uint4 st = ok ? F[id] : make_uint4((uint32_t)kNoLink, 0u, 0u, 0u);
     const bool link = (int)st.x != kNoLink;
     if (link && (int)st.x != kEnd) ctr[0] = DPX_EINVAL;  // not at the end: the links are no permutation
     const bool head = link && (int)st.x == kEnd && (st.w & 2u);
-    // one atomic per wave: the wave's paths take consecutive ranges
+    // one atomic per block (one per wave made the counter the kernel's
+    // bottleneck): the block's paths take consecutive ranges
+    __shared__ uint64_t s_tot[kHeadsBlock / 64], s_base;
+    const int wave = threadIdx.x >> 6;
     const uint64_t len = head ? (uint64_t)st.y + 1u : 0u;
     uint64_t incl = len;
 #pragma unroll
@@ -671,12 +676,19 @@ __global__ __launch_bounds__(256) void k_bc_heads(const uint4* __restrict__ F, u
         const uint64_t t = __shfl_up(incl, d);
         if (lane >= d) incl += t;
     }
-    const uint64_t total = __shfl(incl, 63);
-    if (total == 0) return;
-    uint64_t base = 0;
-    if (lane == 0) base = total > E ? (uint64_t)E + 1u : (uint64_t)atomicAdd(reinterpret_cast<unsigned*>(&ctr[1]), (unsigned)total);
-    base = __shfl(base, 0);
+    if (lane == 63) s_tot[wave] = incl;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint64_t total = 0;
+        for (int w = 0; w < kHeadsBlock / 64; w++) total += s_tot[w];
+        s_base = total == 0 ? 0u
+                 : total > E ? (uint64_t)E + 1u
+                             : (uint64_t)atomicAdd(reinterpret_cast<unsigned*>(&ctr[1]), (unsigned)total);
+    }
+    __syncthreads();
     if (!head) return;
+    uint64_t base = s_base;
+    for (int w = 0; w < wave; w++) base += s_tot[w];
     const uint64_t off = base + incl - len;
     if (off + len > E) {
         ctr[0] = DPX_EINVAL;
@@ -1174,7 +1186,7 @@ extern "C" int dpx_stage_border_chains(const uint8_t* d_graph, int W, int H, uin
         hipLaunchKernelGGL(k_bc_jump<false>, grid, dim3(256), 0, s, cur, oth, E);
         std::swap(cur, oth);
     }
-    hipLaunchKernelGGL(k_bc_heads, grid, dim3(256), 0, s, cur, oth, ctr, E);
+    hipLaunchKernelGGL(k_bc_heads, dim3((E + kHeadsBlock - 1) / kHeadsBlock), dim3(kHeadsBlock), 0, s, cur, oth, ctr, E);
     hipLaunchKernelGGL(k_bc_scatter, grid, dim3(256), 0, s, d_graph, cur, oth, A, rec, ctr, E);
     if (use_lds) {
         hipLaunchKernelGGL(k_bc_scan<true>, dim3(1), dim3(256), lds, s, d_graph, (int)N, rec, A, nullptr, d_nodes,

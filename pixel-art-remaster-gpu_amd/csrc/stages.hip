@@ -609,16 +609,23 @@ __global__ __launch_bounds__(256) void k_bc_init(const uint8_t* __restrict__ g, 
     S[id] = make_uint4((uint32_t)nx, nx >= 0 ? 1u : 0u, nx >= 0 ? (uint32_t)nx : id, id);
 }
 
-// one pointer-jumping round, in -> out; MIN: also the smallest link seen
-template <bool MIN>
+// pointer jumping, in -> out; MIN: also the smallest link seen.  A link's
+// state in `in` covers 2^k steps; composed with the state it points to
+// HOPS times it covers (HOPS + 1) 2^k: HOPS = 1 is one doubling round,
+// HOPS = 3 two rounds and HOPS = 7 three in one launch (dependent reads of
+// `in` in place of launches that each read and write every state).
+template <bool MIN, int HOPS>
 __global__ __launch_bounds__(256) void k_bc_jump(const uint4* __restrict__ in, uint4* __restrict__ out, uint32_t E)
 {
     const uint32_t id = blockIdx.x * 256u + threadIdx.x;
     if (id >= E) return;
     uint4 st = in[id];
-    if ((int)st.x >= 0) {
-        const uint4 x = in[st.x];
-        st = make_uint4(x.x, st.y + x.y, x.z, MIN ? (x.w < st.w ? x.w : st.w) : st.w);
+#pragma unroll
+    for (int h = 0; h < HOPS; h++) {
+        if ((int)st.x >= 0) {
+            const uint4 x = in[st.x];
+            st = make_uint4(x.x, st.y + x.y, x.z, MIN ? (x.w < st.w ? x.w : st.w) : st.w);
+        }
     }
     out[id] = st;
 }
@@ -1176,14 +1183,28 @@ extern "C" int dpx_stage_border_chains(const uint8_t* d_graph, int W, int H, uin
     hipLaunchKernelGGL(k_bc_clear, dim3(1), dim3(64), 0, s, ctr);
     hipLaunchKernelGGL(k_bc_init, grid, dim3(256), 0, s, d_graph, W, H, SA, ctr);
     uint4 *cur = SA, *oth = SB;
-    for (int r = 0; r < rounds; r++) {
-        hipLaunchKernelGGL(k_bc_jump<true>, grid, dim3(256), 0, s, cur, oth, E);
+    for (int r = 0; r < rounds;) {  // (three rounds a launch while three are left)
+        const int m = rounds - r >= 3 ? 3 : rounds - r;
+        if (m == 3)
+            hipLaunchKernelGGL((k_bc_jump<true, 7>), grid, dim3(256), 0, s, cur, oth, E);
+        else if (m == 2)
+            hipLaunchKernelGGL((k_bc_jump<true, 3>), grid, dim3(256), 0, s, cur, oth, E);
+        else
+            hipLaunchKernelGGL((k_bc_jump<true, 1>), grid, dim3(256), 0, s, cur, oth, E);
+        r += m;
         std::swap(cur, oth);
     }
     hipLaunchKernelGGL(k_bc_cut, grid, dim3(256), 0, s, d_graph, W, H, cur, oth, E);
     std::swap(cur, oth);
-    for (int r = 0; r < rounds; r++) {
-        hipLaunchKernelGGL(k_bc_jump<false>, grid, dim3(256), 0, s, cur, oth, E);
+    for (int r = 0; r < rounds;) {
+        const int m = rounds - r >= 3 ? 3 : rounds - r;
+        if (m == 3)
+            hipLaunchKernelGGL((k_bc_jump<false, 7>), grid, dim3(256), 0, s, cur, oth, E);
+        else if (m == 2)
+            hipLaunchKernelGGL((k_bc_jump<false, 3>), grid, dim3(256), 0, s, cur, oth, E);
+        else
+            hipLaunchKernelGGL((k_bc_jump<false, 1>), grid, dim3(256), 0, s, cur, oth, E);
+        r += m;
         std::swap(cur, oth);
     }
     hipLaunchKernelGGL(k_bc_heads, dim3((E + kHeadsBlock - 1) / kHeadsBlock), dim3(kHeadsBlock), 0, s, cur, oth, ctr, E);

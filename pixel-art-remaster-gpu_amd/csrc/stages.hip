@@ -910,8 +910,7 @@ __global__ __launch_bounds__(256) void k_bc_scan(const uint8_t* __restrict__ g, 
                         int nd2[kBcU2];
                         uint64_t bal[kBcU2];
 #pragma unroll
-                        for (int u = 0; u < kBcU2; u++)  // (uniform)
-                            nd2[u] = walk_node(off, L, pos, c + 1 + lane + 64 * u, n);
+                        for (int u = 0; u < kBcU2; u++) nd2[u] = walk_node(off, L, pos, c + 1 + lane + 64 * u, n);
 #pragma unroll
                         for (int u = 0; u < kBcU2; u++) {
                             const int kk = c + 1 + lane + 64 * u;

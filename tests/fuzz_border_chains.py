@@ -38,7 +38,7 @@ for seed in range(int(sys.argv[1]), int(sys.argv[2])):
                         g[j, i] |= 1 << e
                         g[nj, ni] |= 1 << (7 - e)
         if rng.random() < 0.3 and W > 2 and H > 2:  # some internal nodes (90: the four orthogonal links), symmetric
-            for _ in range(int(rng.integers(1, 1 + W * H // 10))):
+            for _ in range(int(rng.integers(1, 2 + W * H // 10))):  # (1 .. W*H/10 + 1: a 3x3 frame gets one)
                 i, j = int(rng.integers(1, W - 1)), int(rng.integers(1, H - 1))
                 old = int(g[j, i])
                 for e in range(8):  # drop the old links' reverses

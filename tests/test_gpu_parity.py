@@ -275,15 +275,22 @@ def run_batch(torch, D, frames, items_spec):
 def test_tall_graph_tiles_match_oracle(gpu, D, O):
     """A batch large enough for full-height graph tiles, with heights for
     which the plan takes tiles of 56 rows rather than 32 (224 = 4 x 56; 100,
-    110 and 50 rows end in a partial tile): sampled items' graphs and pixels
-    equal the oracle's."""
+    110 and 50 rows end in a partial tile), in BGR and RGB, bottom-up and
+    top-down rows, and 4-channel sprites with transparent pixels: sampled
+    items' graphs and pixels equal the oracle's."""
     torch = gpu
     sizes = [(256, 224), (100, 100), (70, 110), (33, 50)]
+    layouts = [0, D.RGB, D.TOPDOWN, D.RGB | D.TOPDOWN]
     frames, spec = [], []
     for k in range(160):
         W, H = sizes[k % 4]
-        frames.append((D.synth_frame((0, 2, 4)[k % 3], 900 + k, W, H, 3), W, H, 3))
-        spec.append((2, False, 0))
+        flags = layouts[(k // 4) % 4]
+        if k % 5 == 4:  # a sprite with an alpha background
+            frames.append((D.synth_frame(1, 900 + k, W, H, 4), W, H, 4))
+            flags |= D.ALPHA
+        else:
+            frames.append((D.synth_frame((0, 2, 4)[k % 3], 900 + k, W, H, 3), W, H, 3))
+        spec.append((2, False, flags))
     # the plan's rule, restated: enough 32-row tiles for full-height tiles,
     # and fewer rows converted (tiles x (rows + 4)) with 56-row tiles
     t32 = sum(-(-W // 64) * -(-H // 32) for _, W, H, _ in frames)
@@ -291,9 +298,9 @@ def test_tall_graph_tiles_match_oracle(gpu, D, O):
     assert t32 >= 512 and t56 * 60 < t32 * 36
     outs, graphs = run_batch(torch, D, frames, spec)
     for k in range(0, 160, 7):
-        img, W, H, _ = frames[k]
-        print("item", k, W, H, flush=True)
-        ref = O.depixelize(img, W, H, scale=2)
+        img, W, H, ch = frames[k]
+        print("item", k, W, H, ch, spec[k][2], flush=True)
+        ref = O.depixelize(img, W, H, scale=2, channels=ch, flags=spec[k][2])
         assert_equal(graphs[k], ref["graph"], f"graph {k}")
         assert_equal(outs[k], ref["rgba"], f"rgba {k}")
 
